@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark: M2L translations/s at P=10, FP64, on one MI355X.
+
+Workload: a batch of 100,000 independent M2L translations of order P=10
+(degrees 0..9, 110 reals per expansion), each with its own source and
+target.  Sources: Re/Im i.i.d. U[-1,1] with Im C_n^0 = 0; shifts: uniform
+directions with |x| ~ U[2,4]; targets accumulate.  Inputs are seeded
+(solidfmm_amd.workload) and resident in HBM before the timed region.
+
+A step is one pass of the plan over the batch (one kernel launch).  W warmup
+steps, then K timed steps between device synchronisations; HIP events on the
+launch stream give the per-step intervals, the isolated batch latency and
+the kernel's average duration for the roofline.
+
+Prints one JSON line.  Only one GPU is supported: --gpus N > 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_M2L = {}  # P -> algorithmic HBM bytes per translation
+
+
+def algorithmic_bytes(p_in, p_out):
+    """Read the source once, read and write the target, plus the 3 FP64
+    components of the shift: 8*[P_in(P_in+1) + 2 P_out(P_out+1)] + 24."""
+    return 8 * (p_in * (p_in + 1) + 2 * p_out * (p_out + 1)) + 24
+
+
+def algorithmic_flops_m2l(P):
+    s2 = sum(k * k for k in range(1, P + 1))
+    t = P * (P + 1) // 2
+    fz = sum(4 * (P - m) * (P - m) for m in range(P))
+    return 4 * (s2 + s2) + 14 * (t + t) + fz
+
+
+def cpu_baseline(items, src, gpu_out, P, threads):
+    """Time the reference's CPU path (oracle/_ref) on the same batch; fall back
+    to the oracle port when the reference library was not built."""
+    import oracle
+    n = len(items)
+    dst = np.zeros(n * P * (P + 1))
+    if oracle.Reference.available():
+        ref = oracle.Reference()
+        secs = ref.translate(1, P, items, src, dst, threads=threads, repeats=5, chunk=0)
+        kind, cores = "reference", threads
+        note = ("vorticle/solidfmm 1.5 single-thread m2l_unchecked on {t} std::threads, targets "
+                "hash-partitioned (multithreaded_translator needs hwloc, absent); full batch, best of 5").format(t=threads)
+        # the reference's AVX-512 lanes 12-15 defect makes its own output unfit
+        # as a checker; compare the GPU against the safe-chunk reference path
+        check = np.zeros_like(dst)
+        ref.translate(1, P, items[:4096], src, check, threads=1, repeats=1, chunk=ref.SAFE_CHUNK)
+        nchk = 4096
+    else:
+        orc = oracle.Oracle()
+        t0 = time.perf_counter()
+        orc.translate(1, items, src, dst)
+        secs = time.perf_counter() - t0
+        kind, cores, note = "port", 1, "oracle FP64 restatement, 1 thread, full batch"
+        check, nchk = dst, 4096
+    got = gpu_out.reshape(n, -1)[:nchk]
+    want = check.reshape(n, -1)[:nchk]
+    err = oracle.degree_errors(got, want, P)
+    return {"value": n / secs, "unit": "translations/s", "cores": cores, "kind": kind, "sample": note}, err
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=100_000)
+    ap.add_argument("--P", type=int, default=10)
+    ap.add_argument("--latency-reps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import solidfmm_amd
+    from solidfmm_amd import workload as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    solidfmm_amd.load()
+    P, n = args.P, args.n
+    seed = 0x5011DF00
+    items, src, dst = W.uniform_batch(W.M2L, P, P, n, seed)
+    ctx = solidfmm_amd.Context(P)
+    plan = solidfmm_amd.Plan(ctx, W.M2L, items)
+    assert plan.launches == 1
+    s = torch.from_numpy(src).cuda()
+    d = torch.zeros(n * P * (P + 1), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    # Correctness leg output: one pass on zeroed targets.
+    plan.execute(s, d, stream)
+    torch.cuda.synchronize()
+    gpu_out = d.cpu().numpy()
+
+    for _ in range(args.warmup):
+        plan.execute(s, d, stream)
+    torch.cuda.synchronize()
+
+    K = args.steps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for k in range(K):
+        plan.execute(s, d, stream)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = ev[0].elapsed_time(ev[K])
+    step_ms = np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(K)])
+
+    # Isolated batch latency: launch, wait, repeat.
+    lat = []
+    for _ in range(args.latency_reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        plan.execute(s, d, stream)
+        b.record(stream)
+        b.synchronize()
+        lat.append(a.elapsed_time(b))
+    lat = np.array(lat)
+
+    ms_per_step = total_ms / K
+    value = n / (ms_per_step * 1e-3)
+    bpt = algorithmic_bytes(P, P)
+    fpt = algorithmic_flops_m2l(P)
+    achieved = value * bpt / 1e9  # GB/s; one launch per step, so this is per launch
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_m2l_p{P}.json")
+    if os.path.exists(tpath) and n == 100_000:
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    line = {
+        "metric": "M2L translations/s at P=10 FP64 on 1 GPU + p99 batch latency at 1e-10 rel err",
+        "value": value,
+        "unit": "translations/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"M2L FP64, P={P} (degrees 0..{P - 1}), {n} independent translations per batch, "
+                               "|shift| in [2,4], seeded",
+                   "P": P, "batch": n, "op": "m2l"},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "step_interval_p50": float(np.percentile(step_ms, 50)),
+                       "step_interval_p99": float(np.percentile(step_ms, 99))},
+        "wall_s": wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": traffic,
+                     "algorithmic_bytes_per_translation": bpt,
+                     "fp64_tflops": value * fpt / 1e12, "fp64_frac": value * fpt / 78.6e12},
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        base, err = cpu_baseline(items, src, gpu_out, P, threads)
+        line["cpu_baseline"] = base
+        line["max_rel_err_vs_reference"] = float(err.max())
+        line["p99_rel_err_vs_reference"] = float(np.percentile(err.max(axis=1), 99))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,128 @@
+/*
+ * solidfmm_gpu.h -- C ABI of the MI355X translation engine.
+ *
+ * Batched M2M / M2L / L2L translations of solid-harmonic expansions, run by
+ * hand-written HIP kernels on one AMD Instinct MI355X (gfx950).  The C++
+ * drop-in headers under include/solidfmm/ (the reference's public API) are
+ * implemented on top of these entry points; other languages bind them
+ * directly.
+ *
+ * Reference interfaces replaced (file:line in vorticle/solidfmm 1.5):
+ *   solid<real> coefficient layout        solidfmm/solid.hpp:57-60
+ *   translation_info<real>                solidfmm/translations.hpp:39-45
+ *   operator_handle / buffer_handle       solidfmm/handles.hpp:33-110
+ *   m2m / m2l / l2l (+ _unchecked)        solidfmm/translations.hpp:103-170
+ *   multithreaded_translator<real>        solidfmm/translations.hpp:47-87
+ *
+ * Conventions
+ *   - Order P means degrees n = 0..P-1; an expansion holds P(P+1) reals:
+ *     Re C_n^m at n(n+1)+m, Im C_n^m at (n+1)^2+m (solid.hpp:59-60).
+ *     Vector-valued expansions repeat that block per dimension.
+ *   - A pool is a device array of reals holding many expansions; an item
+ *     names its source and target by offset (in reals) into the source and
+ *     target pools.  Offsets must be even (16-byte aligned blocks), which
+ *     every packed pool of whole expansions satisfies.
+ *   - Every operator accumulates: target += T(source).  The shift (x,y,z)
+ *     is new centre minus old centre (doc/solidfmm.tex:1954).
+ *   - Entry points return SFMM_OK or a negative SFMM_ERR_* code; the text
+ *     of the last error on a context is sfmm_last_error().  No C++
+ *     exception crosses this boundary.
+ *   - Calls on one context are serialised internally; launches are
+ *     asynchronous on the caller's HIP stream (NULL = the null stream).
+ */
+#ifndef SOLIDFMM_GPU_H
+#define SOLIDFMM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Operators. */
+enum { SFMM_M2M = 0, SFMM_M2L = 1, SFMM_L2L = 2 };
+
+/* Arithmetic of the translation kernels. */
+enum {
+    SFMM_FP64 = 0 /* FP64 throughout (the reference's double path) */
+};
+
+/* Error codes (mirror the reference's exception types). */
+enum {
+    SFMM_OK = 0,
+    SFMM_ERR_LOGIC = -1,       /* std::logic_error: bad argument, dimension mismatch, P == 0 */
+    SFMM_ERR_RANGE = -2,       /* std::out_of_range: order above the context's order */
+    SFMM_ERR_OVERFLOW = -3,    /* std::overflow_error: order too large for the type */
+    SFMM_ERR_ALLOC = -4,       /* std::bad_alloc: host or device allocation failed */
+    SFMM_ERR_HIP = -5,         /* std::runtime_error: a HIP runtime call failed */
+    SFMM_ERR_UNSUPPORTED = -6  /* valid request this build has no kernel for */
+};
+
+/* One translation.  48 bytes, naturally aligned.  Replaces
+ * translation_info<real> (translations.hpp:39-45): pointers become pool
+ * offsets, and the orders travel with the item. */
+typedef struct sfmm_item {
+    uint64_t src;   /* offset (reals) of the source block in the source pool */
+    uint64_t dst;   /* offset (reals) of the target block in the target pool */
+    uint32_t p_in;  /* order of the source expansion */
+    uint32_t p_out; /* order of the target expansion */
+    double x, y, z; /* shift: new centre minus old centre */
+} sfmm_item;
+
+typedef struct sfmm_context sfmm_context;
+typedef struct sfmm_plan sfmm_plan;
+
+/* Highest order this build supports (FP64). */
+size_t sfmm_max_order(void);
+
+/* Context on HIP device `device` for orders up to P.  Replaces
+ * operator_handle<real>(P) (handles.hpp:40, handles.cpp:42-45): P == 0 is
+ * SFMM_ERR_LOGIC, P above sfmm_max_order() is SFMM_ERR_OVERFLOW. */
+int sfmm_create(sfmm_context **ctx, int device, size_t P, int precision);
+void sfmm_destroy(sfmm_context *ctx);
+size_t sfmm_order(const sfmm_context *ctx);
+const char *sfmm_last_error(const sfmm_context *ctx);
+
+/* Plan: a validated, device-resident translation list, grouped into
+ * launches of equal (p_in, p_out) and of pairwise distinct targets.
+ * Replaces the validation and queueing of m2m/m2l/l2l
+ * (translations.cpp:634-1001): every item is checked before anything is
+ * written (orders <= context order, else SFMM_ERR_RANGE; odd offsets
+ * SFMM_ERR_LOGIC).  Items sharing a target are applied in list order. */
+int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, sfmm_plan **plan);
+void sfmm_plan_destroy(sfmm_plan *plan);
+size_t sfmm_plan_launches(const sfmm_plan *plan);
+
+/* Reals of each pool the plan reads or writes: the largest offset+size. */
+void sfmm_plan_extents(const sfmm_plan *plan, size_t *src_len, size_t *dst_len);
+
+/* Run a plan: target pool += T(source pool), asynchronously on `stream`.
+ * Pools are device pointers of src_len / dst_len reals; a pool shorter than
+ * the plan's extent is SFMM_ERR_LOGIC and nothing is launched.  Replaces
+ * multithreaded_translator<real>::m2m/m2l/l2l (translations.hpp:75-81) for
+ * device-resident pools. */
+int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len,
+                      void *stream);
+
+/* Direct launch over device-resident items the caller has already grouped:
+ * all items have orders (p_in, p_out) and pairwise distinct targets.
+ * Replaces m2m/m2l/l2l_unchecked (translations.hpp:154-170): nothing is
+ * validated. */
+int sfmm_launch(sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, const sfmm_item *dev_items,
+                size_t n, const double *src_pool, double *dst_pool, void *stream);
+
+/* Host-pool convenience: copies the pools to the device, runs the list and
+ * copies the target pool back (blocking).  The drop-in C++ API and the
+ * classic single-thread entry points use this path. */
+int sfmm_translate_host(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, const double *src_pool,
+                        size_t src_len, double *dst_pool, size_t dst_len);
+
+/* Blocks until all work the context issued on `stream` is done. */
+int sfmm_synchronize(sfmm_context *ctx, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOLIDFMM_GPU_H */

@@ -1,0 +1,167 @@
+"""ctypes bindings of the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+``Oracle`` wraps liboracle.so, the FP64 restatement of the reference
+algorithm (oracle.c).  ``Reference`` wraps _ref/libsolidfmm_ref.so, the
+reference library compiled from its own sources (Makefile, ref_shim.cpp);
+it exists only where it was built.
+
+Only tests/, the smoke check and the CPU-baseline leg of bench.py may use
+this package.  The product path never imports it.
+
+Functions whose result no independent check pins: none.  The pins are
+listed per function in tests/test_oracle.py.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libsolidfmm_ref.so")
+REF_TREE = "/root/reference"
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def build(reference=True):
+    """Compile liboracle.so, and the reference library when its tree is present."""
+    targets = ["oracle"]
+    if reference and os.path.isdir(os.path.join(REF_TREE, "solidfmm")):
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
+
+
+def _ptr(a, dtype=np.float64):
+    assert a.dtype == dtype and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp if dtype == np.float64 else _fp)
+
+
+class Oracle:
+    def __init__(self, path=LIB):
+        if not os.path.exists(path):
+            build(reference=False)
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.orc_translate.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]
+        L.orc_translate_naive.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]
+        L.orc_harmonic.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, _dp]
+        L.orc_swap_entry.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_folded_matrix.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_euler.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_dot.argtypes = [_dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_dot.restype = ctypes.c_double
+
+    def translate(self, op, items, src, dst, naive=False):
+        """dst += T(src) for every item, in list order (dst is updated in place)."""
+        items = np.ascontiguousarray(items)
+        assert src.dtype == np.float64 and dst.dtype == np.float64
+        f = self.lib.orc_translate_naive if naive else self.lib.orc_translate
+        rc = f(op, items.ctypes.data, len(items), _ptr(src), _ptr(dst))
+        if rc != 0:
+            raise RuntimeError(f"oracle translate failed ({rc})")
+        return dst
+
+    def harmonic(self, kind, P, x, y, z):
+        out = np.zeros(P * (P + 1))
+        if self.lib.orc_harmonic({"R": 0, "S": 1}[kind], P, x, y, z, _ptr(out)) != 0:
+            raise ValueError("bad harmonic request")
+        return out
+
+    def swap_entry(self, n, m, l):
+        v = ctypes.c_double()
+        if self.lib.orc_swap_entry(n, m, l, ctypes.byref(v)) != 0:
+            raise ValueError("bad swap entry")
+        return v.value
+
+    def folded(self, kind, n):
+        out = np.zeros((n + 1) * (n + 1))
+        if self.lib.orc_folded_matrix(kind, n, _ptr(out)) != 0:
+            raise ValueError("bad folded matrix")
+        return out.reshape(n + 1, n + 1)
+
+    def euler(self, x, y, z, k):
+        out = np.zeros(6 * k)
+        self.lib.orc_euler(x, y, z, k, _ptr(out))
+        return out.reshape(6, k)
+
+    def dot(self, a, PA, b, PB):
+        return self.lib.orc_dot(_ptr(a), PA, _ptr(b), PB)
+
+
+class Reference:
+    """The compiled reference (single-thread entry points over host solids)."""
+
+    def __init__(self, path=REF_LIB):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.ref_translate_f64.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                        _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, _dp]
+        L.ref_translate_f32.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                        _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, _dp]
+        L.ref_swap_entry.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
+        L.ref_harmonic.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double, _dp]
+        L.ref_check_errors.argtypes = [ctypes.c_int] + [ctypes.c_size_t] * 5
+
+    @staticmethod
+    def available(path=REF_LIB):
+        return os.path.exists(path)
+
+    # Calls of at most this many items keep every item in SIMD lanes 0-11,
+    # away from the AVX-512 solid2buf defect (microkernel_avx512f.cpp:2182).
+    SAFE_CHUNK = 12
+
+    def translate(self, op, P_op, items, src, dst, threads=1, repeats=1, chunk=SAFE_CHUNK):
+        """dst += T(src) through the reference; returns the best wall time (s).
+
+        ``chunk`` = 0 issues the whole list in one call (full 16-lane
+        batches, as the reference's own callers do).
+        """
+        items = np.ascontiguousarray(items)
+        secs = ctypes.c_double()
+        if src.dtype == np.float64:
+            rc = self.lib.ref_translate_f64(op, P_op, items.ctypes.data, len(items), _ptr(src), _ptr(dst),
+                                            threads, repeats, chunk, ctypes.byref(secs))
+        else:
+            rc = self.lib.ref_translate_f32(op, P_op, items.ctypes.data, len(items), _ptr(src, np.float32),
+                                            _ptr(dst, np.float32), threads, repeats, chunk, ctypes.byref(secs))
+        if rc != 0:
+            raise RuntimeError(f"reference translate failed ({rc})")
+        return secs.value
+
+    def swap_entry(self, n, m, l):
+        v = ctypes.c_double()
+        self.lib.ref_swap_entry(n, m, l, ctypes.byref(v))
+        return v.value
+
+    def harmonic(self, kind, P, x, y, z):
+        out = np.zeros(P * (P + 1))
+        self.lib.ref_harmonic({"R": 0, "S": 1}[kind], P, x, y, z, _ptr(out))
+        return out
+
+    def check_errors(self, op, P_op, P_in, dim_in, P_out, dim_out):
+        return self.lib.ref_check_errors(op, P_op, P_in, dim_in, P_out, dim_out)
+
+
+def degree_errors(got, ref, P):
+    """Per-degree normwise relative error of solids (rows) of order P.
+
+    e_n = ||got_n - ref_n||_2 / ||ref_n||_2 over Re and Im of m = 0..n
+    (the 2(n+1) reals of degree n).  Returns an array (count, P); degrees
+    whose reference norm is zero report the absolute error instead.
+    """
+    got = np.asarray(got).reshape(-1, P * (P + 1))
+    ref = np.asarray(ref).reshape(-1, P * (P + 1))
+    out = np.zeros((got.shape[0], P))
+    for n in range(P):
+        idx = list(range(n * (n + 1), (n + 1) * (n + 2)))
+        d = np.linalg.norm(got[:, idx] - ref[:, idx], axis=1)
+        r = np.linalg.norm(ref[:, idx], axis=1)
+        out[:, n] = np.where(r > 0, d / np.where(r > 0, r, 1.0), d)
+    return out

@@ -1,0 +1,218 @@
+"""ctypes binding of libsolidfmm_gpu.so (the C ABI of include/solidfmm_gpu.h).
+
+The library is built in-tree by solidfmm_amd/build.py.  There is no CPU
+fallback: if the library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .workload import ITEM_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsolidfmm_gpu.so")
+
+SFMM_OK = 0
+SFMM_ERR_LOGIC = -1
+SFMM_ERR_RANGE = -2
+SFMM_ERR_OVERFLOW = -3
+SFMM_ERR_ALLOC = -4
+SFMM_ERR_HIP = -5
+SFMM_ERR_UNSUPPORTED = -6
+
+SFMM_FP64 = 0
+
+# Every function include/solidfmm_gpu.h declares, with its ctypes signature.
+_c = ctypes
+_vp = _c.c_void_p
+SIGNATURES = {
+    "sfmm_max_order": (_c.c_size_t, []),
+    "sfmm_create": (_c.c_int, [_c.POINTER(_vp), _c.c_int, _c.c_size_t, _c.c_int]),
+    "sfmm_destroy": (None, [_vp]),
+    "sfmm_order": (_c.c_size_t, [_vp]),
+    "sfmm_last_error": (_c.c_char_p, [_vp]),
+    "sfmm_plan_create": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
+    "sfmm_plan_destroy": (None, [_vp]),
+    "sfmm_plan_launches": (_c.c_size_t, [_vp]),
+    "sfmm_plan_extents": (None, [_vp, _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
+    "sfmm_plan_execute": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
+    "sfmm_launch": (_c.c_int, [_vp, _c.c_int, _c.c_uint32, _c.c_uint32, _vp, _c.c_size_t, _vp, _vp, _vp]),
+    "sfmm_translate_host": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t]),
+    "sfmm_synchronize": (_c.c_int, [_vp, _vp]),
+}
+
+
+class SolidfmmError(RuntimeError):
+    """Base class; subclasses mirror the reference's exception types."""
+
+
+class LogicError(SolidfmmError, ValueError):
+    """std::logic_error in the reference."""
+
+
+class OutOfRangeError(SolidfmmError, IndexError):
+    """std::out_of_range in the reference."""
+
+
+class OrderOverflowError(SolidfmmError, OverflowError):
+    """std::overflow_error in the reference."""
+
+
+class DeviceAllocError(SolidfmmError, MemoryError):
+    """std::bad_alloc in the reference."""
+
+
+class HipError(SolidfmmError):
+    """A HIP runtime call failed."""
+
+
+class UnsupportedError(SolidfmmError, NotImplementedError):
+    """A valid request this build has no kernel for."""
+
+
+_ERRORS = {
+    SFMM_ERR_LOGIC: LogicError,
+    SFMM_ERR_RANGE: OutOfRangeError,
+    SFMM_ERR_OVERFLOW: OrderOverflowError,
+    SFMM_ERR_ALLOC: DeviceAllocError,
+    SFMM_ERR_HIP: HipError,
+    SFMM_ERR_UNSUPPORTED: UnsupportedError,
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is missing: run `python -m solidfmm_amd.build` first")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _check(rc, ctx=None, what=""):
+    if rc == SFMM_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        raw = load().sfmm_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    raise _ERRORS.get(rc, SolidfmmError)(f"{what}: {msg} (code {rc})")
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return int(stream.cuda_stream)  # torch.cuda.Stream
+
+
+def _dev_ptr(t, dtype="float64"):
+    """(pointer, length) of a contiguous device tensor of reals."""
+    import torch
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.is_contiguous()
+    assert t.dtype == getattr(torch, dtype)
+    return t.data_ptr(), t.numel()
+
+
+class Context:
+    """Device context for orders up to P (operator_handle in the reference)."""
+
+    def __init__(self, P, device=0, precision=SFMM_FP64):
+        lib = load()
+        h = ctypes.c_void_p()
+        rc = lib.sfmm_create(ctypes.byref(h), device, P, precision)
+        _check(rc, None, f"sfmm_create(P={P})")
+        self._h = h
+        self.P = P
+        self.device = device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().sfmm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, op, items):
+        return Plan(self, op, items)
+
+    def launch(self, op, p_in, p_out, dev_items, n, src, dst, stream=None):
+        """Unchecked launch over device-resident items (``dev_items`` is a uint8
+        device tensor holding ``n`` items)."""
+        sp, _ = _dev_ptr(src)
+        dp, _ = _dev_ptr(dst)
+        rc = load().sfmm_launch(self._h, op, p_in, p_out, dev_items.data_ptr(), n, sp, dp, _stream_ptr(stream))
+        _check(rc, self._h, "sfmm_launch")
+
+    def translate_host(self, op, items, src, dst):
+        """dst += T(src) with host (numpy) pools; blocking."""
+        items = np.ascontiguousarray(items, dtype=ITEM_DTYPE)
+        assert src.dtype == np.float64 and dst.dtype == np.float64 and dst.flags["C_CONTIGUOUS"]
+        src = np.ascontiguousarray(src)
+        rc = load().sfmm_translate_host(self._h, op, items.ctypes.data, len(items), src.ctypes.data, src.size,
+                                        dst.ctypes.data, dst.size)
+        _check(rc, self._h, "sfmm_translate_host")
+        return dst
+
+    def synchronize(self, stream=None):
+        _check(load().sfmm_synchronize(self._h, _stream_ptr(stream)), self._h, "sfmm_synchronize")
+
+
+class Plan:
+    """A validated translation list resident on the device."""
+
+    def __init__(self, ctx, op, items):
+        items = np.ascontiguousarray(items, dtype=ITEM_DTYPE)
+        self.ctx = ctx
+        self.op = op
+        self.n = len(items)
+        h = ctypes.c_void_p()
+        rc = load().sfmm_plan_create(ctx.handle, op, items.ctypes.data, len(items), ctypes.byref(h))
+        _check(rc, ctx.handle, "sfmm_plan_create")
+        self._h = h
+
+    @property
+    def launches(self):
+        return load().sfmm_plan_launches(self._h)
+
+    @property
+    def extents(self):
+        a, b = ctypes.c_size_t(), ctypes.c_size_t()
+        load().sfmm_plan_extents(self._h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def execute(self, src, dst, stream=None):
+        """dst += T(src); src, dst are contiguous float64 device tensors."""
+        sp, sn = _dev_ptr(src)
+        dp, dn = _dev_ptr(dst)
+        rc = load().sfmm_plan_execute(self._h, sp, sn, dp, dn, _stream_ptr(stream))
+        _check(rc, self.ctx.handle, "sfmm_plan_execute")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().sfmm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,85 @@
+"""The C ABI library: builds, loads, and exports what include/solidfmm_gpu.h
+declares -- CPU only (no kernel is launched here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import solidfmm_amd
+from solidfmm_amd import _native
+from solidfmm_amd.workload import ITEM_DTYPE
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "solidfmm_gpu.h")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(_native.LIB_PATH):
+        from solidfmm_amd import build
+        build.build(verbose=False)
+    return _native.load()
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfmm_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "sfmm_plan_execute" in names and "sfmm_create" in names
+    assert set(names) == set(_native.SIGNATURES), "ctypes signatures out of sync with the header"
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_item_layout_matches_header():
+    class Item(ctypes.Structure):
+        _fields_ = [("src", ctypes.c_uint64), ("dst", ctypes.c_uint64), ("p_in", ctypes.c_uint32),
+                    ("p_out", ctypes.c_uint32), ("x", ctypes.c_double), ("y", ctypes.c_double),
+                    ("z", ctypes.c_double)]
+    assert ctypes.sizeof(Item) == 48 == ITEM_DTYPE.itemsize
+    for name, (dt, off) in ITEM_DTYPE.fields.items():
+        assert getattr(Item, name).offset == off
+
+
+def test_library_reports_order_limit(lib):
+    assert lib.sfmm_max_order() >= 10
+
+
+def test_product_path_has_no_oracle_dependency():
+    """The product package must not import or load the checkers."""
+    pkg = os.path.join(ROOT, "solidfmm_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hip", ".hpp", ".h")):
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "liboracle" not in text and "_ref" not in text, f
+
+
+def test_python_errors_mirror_reference_types():
+    assert issubclass(solidfmm_amd.LogicError, ValueError)
+    assert issubclass(solidfmm_amd.OutOfRangeError, IndexError)
+    assert issubclass(solidfmm_amd.OrderOverflowError, OverflowError)
+    assert issubclass(solidfmm_amd.DeviceAllocError, MemoryError)
+
+
+def test_workload_prng_is_stable():
+    # SplitMix64 counter stream: pinned values guard against silent changes
+    # to the synthetic inputs.
+    u = solidfmm_amd.workload.uniform01(0x5011DF00, 1, 3)
+    z = np.uint64(0x5011DF00) ^ (np.uint64(1) * np.uint64(0xD1B54A32D192ED03))
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    assert u[0] == float(z >> np.uint64(11)) * 2.0 ** -53
+    assert np.all((u >= 0) & (u < 1))

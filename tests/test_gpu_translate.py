@@ -1,0 +1,178 @@
+"""Parity of the HIP translation kernels with the oracle and the golden
+vectors of the compiled reference -- needs an MI355X.
+
+Every call goes through the C ABI (libsolidfmm_gpu.so).  Tolerances are the
+per-degree normwise relative bounds of tests/helpers.tol_fp64.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from solidfmm_amd import workload as W
+from tests.helpers import assert_parity, load_golden, orders, tol_fp64
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _has_gpu():
+    return torch.cuda.is_available()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not _has_gpu():
+        pytest.fail("GPU tests need a visible GPU")
+
+
+@pytest.fixture(scope="module")
+def sfmm():
+    import solidfmm_amd
+    solidfmm_amd.load()
+    return solidfmm_amd
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def run_gpu(sfmm, op, items, src, dst0, P=None):
+    P = P or max(int(items["p_in"].max(initial=1)), int(items["p_out"].max(initial=1)))
+    ctx = sfmm.Context(P)
+    plan = sfmm.Plan(ctx, op, items)
+    s, d = dev(src), dev(dst0)
+    plan.execute(s, d)
+    torch.cuda.synchronize()
+    return d.cpu().numpy()
+
+
+GOLDEN = load_golden()
+MAXK = 12
+
+
+def _kernel_available(items):
+    pi, po = orders(items)
+    return pi == po and pi <= MAXK
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN))
+def test_golden(sfmm, case):
+    op, items, src, dst0, dst = GOLDEN[case]
+    if not _kernel_available(items):
+        pytest.skip("orders without a kernel in this build")
+    print(f"case={case}", flush=True)
+    got = run_gpu(sfmm, op, items, src, dst0)
+    pi, po = orders(items)
+    assert_parity(got, dst, pi, po, label=case)
+
+
+@pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
+@pytest.mark.parametrize("P", range(1, MAXK + 1))
+def test_oracle_parity_square(sfmm, orc, op, P):
+    n = 200  # several waves and a partial one
+    shift = "m2l" if op == W.M2L else "octant"
+    items, src, dst = W.uniform_batch(op, P, P, n, 0xA5A5 + 17 * P + op, shift=shift)
+    dst0 = W.random_solids(5, 3, n, P).reshape(-1)
+    print(f"op={op} P={P}", flush=True)
+    got = run_gpu(sfmm, op, items, src, dst0)
+    want = orc.translate(op, items, src, dst0.copy())
+    assert_parity(got, want, P, P, label=f"op={op} P={P}")
+
+
+def test_repeated_targets_apply_in_order(sfmm, orc):
+    # 40 items over 3 targets, shuffled, plus zero-order items that do nothing.
+    items, src, _ = W.uniform_batch(W.M2L, 10, 10, 40, 11, distinct_targets=False, n_targets=3)
+    perm = np.argsort(W.uniform01(11, 5, 40))
+    items["dst"] = items["dst"][perm]
+    dst0 = W.random_solids(11, 6, 3, 10).reshape(-1)
+    got = run_gpu(sfmm, W.M2L, items, src, dst0)
+    want = orc.translate(W.M2L, items, src, dst0.copy())
+    # Each target sums ~13 contributions whose cancellation is counted in
+    # the per-target relative error; a lost or doubled update is O(1).
+    assert_parity(got, want, 10, 10, tol=1e-12)
+
+
+def test_zero_shift_and_mixed_lanes(sfmm, orc):
+    for op in (W.M2M, W.L2L):
+        items, src, dst = W.uniform_batch(op, 8, 8, 130, 12, shift="octant")
+        items["x"][::3] = 0.0
+        items["y"][::3] = 0.0
+        items["z"][::3] = 0.0
+        dst0 = W.random_solids(12, 7, 130, 8).reshape(-1)
+        got = run_gpu(sfmm, op, items, src, dst0)
+        want = orc.translate(op, items, src, dst0.copy())
+        assert_parity(got, want, 8, 8)
+
+
+def test_validation_writes_nothing(sfmm):
+    items, src, dst = W.uniform_batch(W.M2L, 6, 6, 10, 13)
+    ctx = sfmm.Context(6)
+    bad = items.copy()
+    bad["p_in"][7] = 7
+    with pytest.raises(sfmm.OutOfRangeError):
+        sfmm.Plan(ctx, W.M2L, bad)
+    odd = items.copy()
+    odd["src"][3] += 1
+    with pytest.raises(sfmm.LogicError):
+        sfmm.Plan(ctx, W.M2L, odd)
+    plan = sfmm.Plan(ctx, W.M2L, items)
+    d = dev(np.zeros(len(dst) - 2))
+    with pytest.raises(sfmm.LogicError):
+        plan.execute(dev(src), d)
+    torch.cuda.synchronize()
+    assert float(d.abs().max()) == 0.0
+
+
+def test_context_errors(sfmm):
+    with pytest.raises(sfmm.LogicError):
+        sfmm.Context(0)
+    with pytest.raises(sfmm.OrderOverflowError):
+        sfmm.Context(10 ** 6)
+
+
+def test_empty_and_zero_order_lists(sfmm):
+    ctx = sfmm.Context(4)
+    empty = np.zeros(0, dtype=W.ITEM_DTYPE)
+    plan = sfmm.Plan(ctx, W.M2L, empty)
+    plan.execute(dev(np.zeros(2)), dev(np.zeros(2)))
+    items, src, dst = W.uniform_batch(W.M2L, 4, 4, 5, 14)
+    items["p_in"] = 0
+    plan = sfmm.Plan(ctx, W.M2L, items)
+    assert plan.launches == 0
+
+
+def test_deterministic(sfmm):
+    items, src, dst = W.uniform_batch(W.M2L, 10, 10, 5000, 15)
+    a = run_gpu(sfmm, W.M2L, items, src, dst)
+    b = run_gpu(sfmm, W.M2L, items, src, dst)
+    assert np.array_equal(a, b)
+
+
+def test_full_size_m2l_p10(sfmm, orc):
+    """The 100,000-translation P=10 workload: a 2,000-item sample against the
+    oracle, plus linearity T(2a - b) = 2 T(a) - T(b) over the whole batch."""
+    n = 100_000
+    items, src, dst = W.uniform_batch(W.M2L, 10, 10, n, 0x5011DF00)
+    src2 = W.random_solids(0x5011DF00, 50, n, 10).reshape(-1)
+    ctx = sfmm.Context(10)
+    plan = sfmm.Plan(ctx, W.M2L, items)
+    assert plan.launches == 1
+    outs = []
+    for s in (src, src2, 2 * src - src2):
+        d = dev(np.zeros_like(dst))
+        plan.execute(dev(s), d)
+        outs.append(d.cpu().numpy())
+    a, b, c = outs
+    # Scale of the operands, not of their difference, which can cancel.
+    def norms(v):  # per-degree 2-norms (degree_errors against zero)
+        return oracle.degree_errors(v, np.zeros_like(v), 10)
+    lin = (norms(c - (2 * a - b)) / (2 * norms(a) + norms(b))).max(axis=1)
+    # internal cancellation inside single translations leaves outliers
+    assert np.percentile(lin, 99) <= 1e-13 and lin.max() <= 1e-12, (np.percentile(lin, 99), lin.max())
+    idx = np.linspace(0, n - 1, 2000).astype(np.int64)
+    sub = items[idx].copy()
+    sub["dst"] = np.arange(len(idx), dtype=np.uint64) * np.uint64(110)
+    want = orc.translate(W.M2L, sub, src, np.zeros(len(idx) * 110))
+    got = a.reshape(n, 110)[idx]
+    assert_parity(got, want, 10, 10, label="sample")
