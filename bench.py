@@ -71,6 +71,29 @@ def cpu_baseline(items, src, gpu_out, P, threads):
     return {"value": n / secs, "unit": "translations/s", "cores": cores, "kind": kind, "sample": note}, err
 
 
+def copy_mode(args):
+    """Plain device-to-device copy of known size: reads and writes
+    args.copy_mib MiB per launch."""
+    import torch
+    n = args.copy_mib * 2 ** 20 // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    reps = max(1, args.steps)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    nbytes = 2 * n * 8
+    print(json.dumps({"mode": "copy", "bytes_per_launch": nbytes, "read_bytes": n * 8, "write_bytes": n * 8,
+                      "ms_per_launch": ms, "GBps": nbytes / (ms * 1e-3) / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +104,14 @@ def main():
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", choices=["m2l", "copy"], default="m2l",
+                    help="copy: time a plain device copy of --copy-mib MiB (HBM reference and the "
+                         "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
+
+    if args.mode == "copy":
+        return copy_mode(args)
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
