@@ -146,30 +146,25 @@ __device__ __forceinline__ void rot_bwd1(const double *c, const double *s, doubl
     });
 }
 
-// Degree N of an expansion: 2(N+1) contiguous reals at N(N+1), 16-byte
-// aligned when the block is.
-template <int N>
-__device__ __forceinline__ void load_degree(const double *__restrict__ blk, double (&re)[N + 1], double (&im)[N + 1])
+// Issue the loads of degree N of a block into v[0..N] (16-byte pieces).
+// Callers place a scheduling barrier after it, so the loads stay where they
+// were issued and their latency overlaps the work that follows.
+template <int N, int CAP>
+__device__ __forceinline__ void fetch_degree(const double *__restrict__ blk, double2 (&v)[CAP])
 {
+    static_assert(N < CAP, "prefetch buffer too small");
     const double2 *q = reinterpret_cast<const double2 *>(blk + N * (N + 1));
-    double a[2 * (N + 1)];
     static_for<0, N + 1>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        const double2 v = q[j];
-        a[2 * j] = v.x;
-        a[2 * j + 1] = v.y;
-    });
-    static_for<0, N + 1>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        re[m] = a[m];
-        im[m] = a[N + 1 + m];
+        v[j] = q[j];
     });
 }
 
-// target degree N += (re, im)   (buf2solid, microkernel_generic.cpp:361-380)
-template <int N>
-__device__ __forceinline__ void add_degree(double *__restrict__ blk, const double (&re)[N + 1],
-                                           const double (&im)[N + 1])
+// target degree N = old + (re, im), old prefetched by fetch_degree
+// (buf2solid, microkernel_generic.cpp:361-380).
+template <int N, int CAP>
+__device__ __forceinline__ void add_store_degree(double *__restrict__ blk, const double2 (&old)[CAP],
+                                                 const double (&re)[N + 1], const double (&im)[N + 1])
 {
     double2 *q = reinterpret_cast<double2 *>(blk + N * (N + 1));
     double a[2 * (N + 1)];
@@ -178,18 +173,31 @@ __device__ __forceinline__ void add_degree(double *__restrict__ blk, const doubl
         a[m] = re[m];
         a[N + 1 + m] = im[m];
     });
-    double2 v[N + 1];
     static_for<0, N + 1>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        v[j] = q[j];
-    });
-    static_for<0, N + 1>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        v[j].x += a[2 * j];
-        v[j].y += a[2 * j + 1];
-        q[j] = v[j];
+        double2 v = old[j];
+        v.x += a[2 * j];
+        v.y += a[2 * j + 1];
+        q[j] = v;
     });
 }
+
+// Degree N of a whole block held as 16-byte pieces b[0..P(P+1)/2).
+template <int N, int CAP>
+__device__ __forceinline__ void unpack_block_degree(const double2 (&b)[CAP], double (&re)[N + 1],
+                                                    double (&im)[N + 1])
+{
+    static_assert(N * (N + 1) / 2 + N < CAP, "degree outside the block");
+    static_for<0, N + 1>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        constexpr int kr = N * (N + 1) + m;          // Re C_N^m
+        constexpr int ki = (N + 1) * (N + 1) + m;    // Im C_N^m
+        re[m] = (kr & 1) ? b[kr / 2].y : b[kr / 2].x;
+        im[m] = (ki & 1) ? b[ki / 2].y : b[ki / 2].x;
+    });
+}
+
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
 // Forward pipelines: M-type (M2M, M2L; translations.cpp:36-85) uses the
 // folded blocks, L-type (L2L; translations.cpp:189-239) their transposes.
@@ -269,6 +277,20 @@ __global__ void __launch_bounds__(64)
     Rot<PM> R;
     euler_powers<PM>(ca1, sa1, cb1, sb1, R);
 
+    // Whole source block in flight at once.  Its registers are handed over
+    // degree by degree to the rotated triangle, so this costs no peak.
+    double2 sv[tri(0, PI)];
+    {
+        const double2 *q = reinterpret_cast<const double2 *>(S);
+        static_for<0, tri(0, PI)>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            sv[j] = q[j];
+        });
+    }
+    // Target degrees are prefetched one ahead of their use (tg); the
+    // scheduling fences keep those loads where they are issued.
+    double2 tg[PO];
+
     // Rotated triangle: T[tri(m,K)] holds order m of degree K.
     double tre[tri(0, PI)], tim[tri(0, PI)];
 
@@ -277,8 +299,12 @@ __global__ void __launch_bounds__(64)
         double s = rinv;
         static_for<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
+            if constexpr (n == PI - 1) {
+                fetch_degree<PO - 1>(D, tg);
+                sched_fence();
+            }
             double re[n + 1], im[n + 1];
-            load_degree<n>(S, re, im);
+            unpack_block_degree<n>(sv, re, im);
             forward<n, false, PM>(R, s, re, im);
             static_for<0, n + 1>([&](auto mc) {
                 constexpr int m = decltype(mc)::value;
@@ -298,6 +324,10 @@ __global__ void __launch_bounds__(64)
         // (translations.cpp:283-296, zm2l microkernel_generic.cpp:216-255).
         static_for_down<0, PO>([&](auto Nc) {
             constexpr int N = decltype(Nc)::value;
+            double2 cur[PO];
+            static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
+            if constexpr (N > 0) fetch_degree<N - 1>(D, tg);
+            sched_fence();
             double re[N + 1], im[N + 1];
             static_for<0, N + 1>([&](auto mc) {
                 constexpr int m = decltype(mc)::value;
@@ -323,7 +353,7 @@ __global__ void __launch_bounds__(64)
                 im[m] = ai;
             });
             backward<N, true, PM>(R, rp[N], re, im);
-            add_degree<N>(D, re, im);
+            add_store_degree<N>(D, cur, re, im);
         });
     } else if constexpr (OP == SFMM_M2M) {
         // Ascending degrees; output degree n needs input degrees K <= n
@@ -331,11 +361,19 @@ __global__ void __launch_bounds__(64)
         // M_N^m = sum_{K<=N} r^{N-K}/(N-K)! T_K^m (translations.cpp:363-377).
         double s = rinv;   // forward scale r^-(n+1)
         double sb = r;     // backward scale r^(N+1) (translations.cpp:129)
+        fetch_degree<0>(D, tg);
+        sched_fence();
         static_for<0, PM>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
+            double2 cur[PO];
+            if constexpr (n < PO) {
+                static_for<0, n + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
+                if constexpr (n + 1 < PO) fetch_degree<n + 1>(D, tg);
+                sched_fence();
+            }
             if constexpr (n < PI) {
                 double re[n + 1], im[n + 1];
-                load_degree<n>(S, re, im);
+                unpack_block_degree<n>(sv, re, im);
                 forward<n, false, PM>(R, s, re, im);
                 static_for<0, n + 1>([&](auto mc) {
                     constexpr int m = decltype(mc)::value;
@@ -368,7 +406,7 @@ __global__ void __launch_bounds__(64)
                     im[m] = ai;
                 });
                 backward<N, false, PM>(R, sb, re, im);
-                add_degree<N>(D, re, im);
+                add_store_degree<N>(D, cur, re, im);
                 sb *= r;
             }
         });
@@ -389,11 +427,15 @@ __global__ void __launch_bounds__(64)
             constexpr int n = decltype(nc)::value;
             bs[n] = bs[n - 1] * rinv;
         });
+        if constexpr (PE > 0) {
+            fetch_degree<PE - 1>(D, tg);
+            sched_fence();
+        }
         static_for_down<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             {
                 double re[n + 1], im[n + 1];
-                load_degree<n>(S, re, im);
+                unpack_block_degree<n>(sv, re, im);
                 forward<n, true, PM>(R, fs[n], re, im);
                 static_for<0, n + 1>([&](auto mc) {
                     constexpr int m = decltype(mc)::value;
@@ -403,6 +445,10 @@ __global__ void __launch_bounds__(64)
             }
             if constexpr (n < PE) {
                 constexpr int N = n;
+                double2 cur[PO];
+                static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
+                if constexpr (N > 0) fetch_degree<N - 1>(D, tg);
+                sched_fence();
                 double re[N + 1], im[N + 1];
                 static_for<0, N + 1>([&](auto mc) {
                     constexpr int m = decltype(mc)::value;
@@ -422,7 +468,7 @@ __global__ void __launch_bounds__(64)
                     im[m] = ai;
                 });
                 backward<N, true, PM>(R, bs[N], re, im);
-                add_degree<N>(D, re, im);
+                add_store_degree<N>(D, cur, re, im);
             }
         });
     }
