@@ -1,10 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark: M2L translations/s at P=10, FP64, on one MI355X.
 
-Workload: a batch of 100,000 independent M2L translations of order P=10
-(degrees 0..9, 110 reals per expansion), each with its own source and
+Default workload: a batch of 100,000 independent M2L translations of order
+P=10 (degrees 0..9, 110 reals per expansion), each with its own source and
 target.  Sources: Re/Im i.i.d. U[-1,1] with Im C_n^0 = 0; shifts: uniform
-directions with |x| ~ U[2,4]; targets accumulate.  Inputs are seeded
+directions with |x| ~ U[2,4] (M2L) or the octant offsets (+-0.5)^3 scaled by
+U[0.5,1] (M2M, L2L); targets accumulate.  Inputs are seeded
 (solidfmm_amd.workload) and resident in HBM before the timed region.
 
 A step is one pass of the plan over the batch (one kernel launch).  W warmup
@@ -12,6 +13,8 @@ steps, then K timed steps between device synchronisations; HIP events on the
 launch stream give the per-step intervals, the isolated batch latency and
 the kernel's average duration for the roofline.
 
+--op / --P / --n / --kernels select other workloads (e.g. --op m2m --P 16
+--n 1000000 for the M2M/L2L configuration); the default line is the headline.
 Prints one JSON line.  Only one GPU is supported: --gpus N > 1 exits.
 """
 import argparse
@@ -25,7 +28,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-BYTES_PER_M2L = {}  # P -> algorithmic HBM bytes per translation
+OPS = {"m2m": 0, "m2l": 1, "l2l": 2}
+KERNELS = {"auto": 0, "lane": 1, "mfma": 2}
 
 
 def algorithmic_bytes(p_in, p_out):
@@ -34,40 +38,50 @@ def algorithmic_bytes(p_in, p_out):
     return 8 * (p_in * (p_in + 1) + 2 * p_out * (p_out + 1)) + 24
 
 
-def algorithmic_flops_m2l(P):
+def algorithmic_flops(op, P):
+    """4 S2 per side for the swaps ((n+1)^2 FMA per swap, 4 swaps per
+    degree), 14 T per side for the rotations, plus the z-translation:
+    Hankel (M2L) or triangular Toeplitz (M2M, L2L) sums."""
     s2 = sum(k * k for k in range(1, P + 1))
     t = P * (P + 1) // 2
-    fz = sum(4 * (P - m) * (P - m) for m in range(P))
+    if op == "m2l":
+        fz = sum(4 * (P - m) * (P - m) for m in range(P))
+    else:
+        fz = sum(2 * (P - m) * (P - m + 1) for m in range(P))
     return 4 * (s2 + s2) + 14 * (t + t) + fz
 
 
-def cpu_baseline(items, src, gpu_out, P, threads):
+def cpu_baseline(op, items, src, gpu_out, P, threads):
     """Time the reference's CPU path (oracle/_ref) on the same batch; fall back
     to the oracle port when the reference library was not built."""
     import oracle
     n = len(items)
+    opi = OPS[op]
     dst = np.zeros(n * P * (P + 1))
     if oracle.Reference.available():
         ref = oracle.Reference()
-        secs = ref.translate(1, P, items, src, dst, threads=threads, repeats=5, chunk=0)
+        secs = ref.translate(opi, P, items, src, dst, threads=threads, repeats=5, chunk=0)
         kind, cores = "reference", threads
-        note = ("vorticle/solidfmm 1.5 single-thread m2l_unchecked on {t} std::threads, targets "
-                "hash-partitioned (multithreaded_translator needs hwloc, absent); full batch, best of 5").format(t=threads)
-        # the reference's AVX-512 lanes 12-15 defect makes its own output unfit
-        # as a checker; compare the GPU against the safe-chunk reference path
-        check = np.zeros_like(dst)
-        ref.translate(1, P, items[:4096], src, check, threads=1, repeats=1, chunk=ref.SAFE_CHUNK)
-        nchk = 4096
+        note = (f"vorticle/solidfmm 1.5 single-thread {op}_unchecked on {threads} std::threads, targets "
+                "hash-partitioned (multithreaded_translator needs hwloc, absent); whole batch, best of 5")
+        # The reference's AVX-512 lanes 12-15 defect (microkernel_avx512f.cpp:2182)
+        # makes its whole-batch output unfit as a checker; check the GPU
+        # against the reference in calls of <= 12 items instead.
+        nchk = min(n, 4096)
+        check = np.zeros(nchk * P * (P + 1))
+        sub = items[:nchk].copy()
+        sub["dst"] = np.arange(nchk, dtype=np.uint64) * np.uint64(P * (P + 1))
+        ref.translate(opi, P, sub, src, check, threads=1, repeats=1, chunk=ref.SAFE_CHUNK)
     else:
         orc = oracle.Oracle()
         t0 = time.perf_counter()
-        orc.translate(1, items, src, dst)
+        orc.translate(opi, items, src, dst)
         secs = time.perf_counter() - t0
-        kind, cores, note = "port", 1, "oracle FP64 restatement, 1 thread, full batch"
-        check, nchk = dst, 4096
+        kind, cores, note = "port", 1, "oracle FP64 restatement, 1 thread, whole batch"
+        nchk = min(n, 4096)
+        check = dst[: nchk * P * (P + 1)]
     got = gpu_out.reshape(n, -1)[:nchk]
-    want = check.reshape(n, -1)[:nchk]
-    err = oracle.degree_errors(got, want, P)
+    err = oracle.degree_errors(got, check.reshape(nchk, -1), P)
     return {"value": n / secs, "unit": "translations/s", "cores": cores, "kind": kind, "sample": note}, err
 
 
@@ -99,24 +113,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--op", choices=list(OPS), default="m2l")
     ap.add_argument("--n", type=int, default=100_000)
     ap.add_argument("--P", type=int, default=10)
+    ap.add_argument("--kernels", choices=list(KERNELS), default="auto")
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["m2l", "copy"], default="m2l",
+    ap.add_argument("--mode", choices=["translate", "copy"], default="translate",
                     help="copy: time a plain device copy of --copy-mib MiB (HBM reference and the "
                          "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
 
-    if args.mode == "copy":
-        return copy_mode(args)
-
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
         sys.exit(2)
+    if args.mode == "copy":
+        return copy_mode(args)
 
     import torch
 
@@ -125,17 +140,18 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     solidfmm_amd.load()
-    P, n = args.P, args.n
+    P, n, op = args.P, args.n, args.op
     seed = 0x5011DF00
-    items, src, dst = W.uniform_batch(W.M2L, P, P, n, seed)
+    items, src, dst = W.uniform_batch(OPS[op], P, P, n, seed, shift="m2l" if op == "m2l" else "octant")
     ctx = solidfmm_amd.Context(P)
-    plan = solidfmm_amd.Plan(ctx, W.M2L, items)
+    ctx.set_kernels(KERNELS[args.kernels])
+    plan = solidfmm_amd.Plan(ctx, OPS[op], items)
     assert plan.launches == 1
     s = torch.from_numpy(src).cuda()
     d = torch.zeros(n * P * (P + 1), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
 
-    # Correctness leg output: one pass on zeroed targets.
+    # One pass on zeroed targets for the error check against the reference.
     plan.execute(s, d, stream)
     torch.cuda.synchronize()
     gpu_out = d.cpu().numpy()
@@ -167,20 +183,23 @@ def main():
         b.record(stream)
         b.synchronize()
         lat.append(a.elapsed_time(b))
-    lat = np.array(lat)
+    lat = np.array(lat) if lat else np.zeros(1)
 
     ms_per_step = total_ms / K
     value = n / (ms_per_step * 1e-3)
     bpt = algorithmic_bytes(P, P)
-    fpt = algorithmic_flops_m2l(P)
-    achieved = value * bpt / 1e9  # GB/s; one launch per step, so this is per launch
+    fpt = algorithmic_flops(op, P)
+    achieved = value * bpt / 1e9  # GB/s; one launch per step, so also per launch
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_m2l_p{P}.json")
-    if os.path.exists(tpath) and n == 100_000:
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{op}_p{P}_n{n}.json")
+    if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("bytes_per_launch")
+    headline = op == "m2l"
+    metric = ("M2L translations/s at P=10 FP64 on 1 GPU + p99 batch latency at 1e-10 rel err" if headline
+              else f"{op.upper()} translations/s at P={P} FP64 on 1 GPU")
     line = {
-        "metric": "M2L translations/s at P=10 FP64 on 1 GPU + p99 batch latency at 1e-10 rel err",
+        "metric": metric,
         "value": value,
         "unit": "translations/s",
         "n_gpus": 1,
@@ -192,9 +211,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"M2L FP64, P={P} (degrees 0..{P - 1}), {n} independent translations per batch, "
-                               "|shift| in [2,4], seeded",
-                   "P": P, "batch": n, "op": "m2l"},
+        "config": {"workload": f"{op.upper()} FP64, P={P} (degrees 0..{P - 1}), {n} independent translations "
+                               f"per batch, " + ("|shift| in [2,4]" if op == "m2l" else "octant shifts")
+                               + ", seeded",
+                   "P": P, "batch": n, "op": op, "kernels": args.kernels},
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "step_interval_p50": float(np.percentile(step_ms, 50)),
                        "step_interval_p99": float(np.percentile(step_ms, 99))},
@@ -206,10 +226,12 @@ def main():
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        base, err = cpu_baseline(items, src, gpu_out, P, threads)
+        base, err = cpu_baseline(op, items, src, gpu_out, P, threads)
         line["cpu_baseline"] = base
-        line["max_rel_err_vs_reference"] = float(err.max())
-        line["p99_rel_err_vs_reference"] = float(np.percentile(err.max(axis=1), 99))
+        per_item = err.max(axis=1)
+        line["rel_err_vs_reference"] = {"p50": float(np.percentile(per_item, 50)),
+                                        "p99": float(np.percentile(per_item, 99)),
+                                        "max": float(per_item.max())}
     print(json.dumps(line), flush=True)
 
 

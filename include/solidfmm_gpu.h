@@ -48,6 +48,13 @@ enum {
     SFMM_FP64 = 0 /* FP64 throughout (the reference's double path) */
 };
 
+/* Kernel families (sfmm_set_kernels). */
+enum {
+    SFMM_KERNELS_AUTO = 0, /* per order: the faster family that has a kernel */
+    SFMM_KERNELS_LANE = 1, /* one lane per translation, FP64 VALU (orders <= 12) */
+    SFMM_KERNELS_MFMA = 2  /* 16 translations per wave, swaps on FP64 MFMA (orders <= 24) */
+};
+
 /* Error codes (mirror the reference's exception types). */
 enum {
     SFMM_OK = 0,
@@ -83,6 +90,10 @@ int sfmm_create(sfmm_context **ctx, int device, size_t P, int precision);
 void sfmm_destroy(sfmm_context *ctx);
 size_t sfmm_order(const sfmm_context *ctx);
 const char *sfmm_last_error(const sfmm_context *ctx);
+
+/* Kernel family used by plans created afterwards on this context (a tuning
+ * knob with no counterpart in the reference; results agree to rounding). */
+int sfmm_set_kernels(sfmm_context *ctx, int family);
 
 /* Plan: a validated, device-resident translation list, grouped into
  * launches of equal (p_in, p_out) and of pairwise distinct targets.

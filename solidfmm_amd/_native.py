@@ -23,6 +23,8 @@ SFMM_ERR_UNSUPPORTED = -6
 
 SFMM_FP64 = 0
 
+KERNELS_AUTO, KERNELS_LANE, KERNELS_MFMA = 0, 1, 2
+
 # Every function include/solidfmm_gpu.h declares, with its ctypes signature.
 _c = ctypes
 _vp = _c.c_void_p
@@ -32,6 +34,7 @@ SIGNATURES = {
     "sfmm_destroy": (None, [_vp]),
     "sfmm_order": (_c.c_size_t, [_vp]),
     "sfmm_last_error": (_c.c_char_p, [_vp]),
+    "sfmm_set_kernels": (_c.c_int, [_vp, _c.c_int]),
     "sfmm_plan_create": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_plan_destroy": (None, [_vp]),
     "sfmm_plan_launches": (_c.c_size_t, [_vp]),
@@ -153,6 +156,10 @@ class Context:
 
     def plan(self, op, items):
         return Plan(self, op, items)
+
+    def set_kernels(self, family):
+        """Kernel family for plans made afterwards: KERNELS_AUTO, _LANE or _MFMA."""
+        _check(load().sfmm_set_kernels(self._h, family), self._h, "sfmm_set_kernels")
 
     def launch(self, op, p_in, p_out, dev_items, n, src, dst, stream=None):
         """Unchecked launch over device-resident items (``dev_items`` is a uint8

@@ -26,7 +26,7 @@ COMMON = [
 
 # Translation units; the kernel instantiations are split so they compile in
 # parallel.
-SOURCES = ["translate.hip", "capi.cpp"]
+SOURCES = ["translate.hip", "mfma_m2m.hip", "mfma_m2l.hip", "mfma_l2l.hip", "mfma_tables.cpp", "capi.cpp"]
 
 
 def _compile(src):

@@ -24,6 +24,7 @@ struct sfmm_context {
     int device = 0;
     size_t P = 0;
     int precision = SFMM_FP64;
+    int family = SFMM_KERNELS_AUTO;
     std::string err;
     std::mutex mu;
 };
@@ -35,6 +36,7 @@ struct sfmm_plan {
     };
     sfmm_context *ctx = nullptr;
     int op = 0;
+    int family = SFMM_KERNELS_AUTO;
     sfmm_item *d_items = nullptr;
     size_t n_items = 0;
     size_t src_extent = 0, dst_extent = 0;
@@ -80,7 +82,7 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
         if ((it.src | it.dst) & 1u)
             return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: expansion offsets must be even (16-byte aligned)");
         if (it.p_in == 0 || it.p_out == 0) continue;
-        if (!sfmm::have_kernel(op, it.p_in, it.p_out))
+        if (!sfmm::have_kernel(op, it.p_in, it.p_out, ctx->family))
             return fail(ctx, SFMM_ERR_UNSUPPORTED,
                         std::string("solidfmm: no kernel for ") + op_name(op) + " with p_in=" +
                             std::to_string(it.p_in) + ", p_out=" + std::to_string(it.p_out));
@@ -136,6 +138,16 @@ size_t sfmm_order(const sfmm_context *ctx) { return ctx ? ctx->P : 0; }
 
 const char *sfmm_last_error(const sfmm_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+int sfmm_set_kernels(sfmm_context *ctx, int family)
+{
+    if (!ctx) return SFMM_ERR_LOGIC;
+    if (family < SFMM_KERNELS_AUTO || family > SFMM_KERNELS_MFMA)
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown kernel family");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->family = family;
+    return SFMM_OK;
+}
+
 int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm_plan **out)
 {
     if (!ctx || !out) return SFMM_ERR_LOGIC;
@@ -151,6 +163,7 @@ int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *items, size_t n
     if (!plan) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
     plan->ctx = ctx;
     plan->op = op;
+    plan->family = ctx->family;
     plan->n_items = ordered.size();
     plan->launches = std::move(launches);
     for (const sfmm_item &it : ordered) {
@@ -202,7 +215,7 @@ int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, d
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     for (const auto &L : plan->launches) {
         e = sfmm::launch_translate(plan->op, L.p_in, L.p_out, plan->d_items + L.offset, L.count, src_pool, dst_pool,
-                                   static_cast<hipStream_t>(stream));
+                                   static_cast<hipStream_t>(stream), plan->family);
         if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
     }
     return SFMM_OK;
@@ -216,10 +229,12 @@ int sfmm_launch(sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, const 
     if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
     if (p_in > ctx->P || p_out > ctx->P) return fail(ctx, SFMM_ERR_RANGE, "solidfmm: orders exceeding operator data.");
     if (n == 0 || p_in == 0 || p_out == 0) return SFMM_OK;
-    if (!sfmm::have_kernel(op, p_in, p_out)) return fail(ctx, SFMM_ERR_UNSUPPORTED, "solidfmm: no kernel for orders");
+    if (!sfmm::have_kernel(op, p_in, p_out, ctx->family))
+        return fail(ctx, SFMM_ERR_UNSUPPORTED, "solidfmm: no kernel for orders");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    e = sfmm::launch_translate(op, p_in, p_out, dev_items, n, src_pool, dst_pool, static_cast<hipStream_t>(stream));
+    e = sfmm::launch_translate(op, p_in, p_out, dev_items, n, src_pool, dst_pool, static_cast<hipStream_t>(stream),
+                               ctx->family);
     if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
     return SFMM_OK;
 }

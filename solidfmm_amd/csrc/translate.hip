@@ -24,6 +24,7 @@
 #include <cstdint>
 
 #include "../../include/solidfmm_gpu.h"
+#include "mfma_tables.hpp"
 #include "tables.hpp"
 #include "translate.hpp"
 
@@ -514,24 +515,43 @@ static const SquareTable &square_table()
     return t;
 }
 
-bool have_kernel(int op, uint32_t p_in, uint32_t p_out)
+// Orders up to this use the lane kernels under kAuto; the MFMA kernels
+// take the rest (the lane kernels spill above it).
+constexpr uint32_t kLaneAutoMax = 10;
+
+bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family)
 {
-    if (op < 0 || op > 2) return false;
-    return p_in == p_out && p_in >= 1 && p_in <= kSquare;
+    if (op < 0 || op > 2 || p_in != p_out || p_in < 1) return false;
+    switch (family) {
+    case kLaneKernels: return p_in <= uint32_t(kSquare);
+    case kMfmaKernels: return p_in >= uint32_t(mfma::kMfmaMin) && p_in <= uint32_t(mfma::kMfmaMax);
+    default: return p_in <= uint32_t(kSquare) || p_in <= uint32_t(mfma::kMfmaMax);
+    }
 }
 
-size_t max_kernel_order() { return kSquare; }
+size_t max_kernel_order() { return size_t(mfma::kMfmaMax); }
 
 hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_item *items, size_t n,
-                            const double *src, double *dst, hipStream_t stream)
+                            const double *src, double *dst, hipStream_t stream, int family)
 {
-    if (!have_kernel(op, p_in, p_out)) return hipErrorInvalidValue;
+    if (!have_kernel(op, p_in, p_out, family)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
-    kernel_fn f = square_table().f[op][p_in - 1];
-    const uint32_t block = 64;
-    const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
-    hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n));
-    return hipGetLastError();
+    if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const bool lane = family == kLaneKernels || (family == kAuto && p_in <= kLaneAutoMax);
+    if (lane) {
+        kernel_fn f = square_table().f[op][p_in - 1];
+        const uint32_t block = 64;
+        const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
+        hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n));
+        return hipGetLastError();
+    }
+    const double *frag = mfma::device_table();
+    if (!frag) return hipErrorOutOfMemory;
+    switch (op) {
+    case SFMM_M2M: return mfma::launch_m2m(p_in, items, n, src, dst, frag, stream);
+    case SFMM_M2L: return mfma::launch_m2l(p_in, items, n, src, dst, frag, stream);
+    default: return mfma::launch_l2l(p_in, items, n, src, dst, frag, stream);
+    }
 }
 
 }  // namespace sfmm

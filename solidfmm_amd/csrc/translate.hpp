@@ -10,15 +10,29 @@
 
 namespace sfmm {
 
-// True when this build has a kernel for (op, p_in, p_out).
-bool have_kernel(int op, uint32_t p_in, uint32_t p_out);
+// True when this build has a kernel for (op, p_in, p_out) in `family`.
+bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family = 0);
 
 // Highest order with a kernel.
 size_t max_kernel_order();
 
+// Kernel families: one lane per translation (FP64 VALU, orders up to 12)
+// and one wave per 16 translations (swaps on the FP64 matrix cores).
+enum Family { kAuto = 0, kLaneKernels = 1, kMfmaKernels = 2 };
+
+namespace mfma {
+inline constexpr int kMfmaMin = 1;
+inline constexpr int kMfmaMax = 24;
+hipError_t launch_m2m(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
+hipError_t launch_m2l(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
+hipError_t launch_l2l(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
+}  // namespace mfma
+
 // Enqueue dst += T(src) for n device-resident items, all of orders
 // (p_in, p_out) and with pairwise distinct targets.
+// `family` picks the kernel family (kAuto: the lane kernels where they
+// exist, else the MFMA kernels).
 hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_item *items, size_t n,
-                            const double *src, double *dst, hipStream_t stream);
+                            const double *src, double *dst, hipStream_t stream, int family = kAuto);
 
 }  // namespace sfmm

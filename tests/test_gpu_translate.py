@@ -37,9 +37,10 @@ def dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
 
 
-def run_gpu(sfmm, op, items, src, dst0, P=None):
+def run_gpu(sfmm, op, items, src, dst0, P=None, family=0):
     P = P or max(int(items["p_in"].max(initial=1)), int(items["p_out"].max(initial=1)))
     ctx = sfmm.Context(P)
+    ctx.set_kernels(family)
     plan = sfmm.Plan(ctx, op, items)
     s, d = dev(src), dev(dst0)
     plan.execute(s, d)
@@ -48,12 +49,14 @@ def run_gpu(sfmm, op, items, src, dst0, P=None):
 
 
 GOLDEN = load_golden()
-MAXK = 12
+LANE_MAX = 12   # one lane per translation
+MFMA_MAX = 24   # 16 translations per wave, swaps on the matrix cores
+LANE, MFMA = 1, 2
 
 
 def _kernel_available(items):
     pi, po = orders(items)
-    return pi == po and pi <= MAXK
+    return pi == po and pi <= MFMA_MAX
 
 
 @pytest.mark.parametrize("case", sorted(GOLDEN))
@@ -67,17 +70,20 @@ def test_golden(sfmm, case):
     assert_parity(got, dst, pi, po, label=case)
 
 
+SQUARE_CASES = [(LANE, P) for P in range(1, LANE_MAX + 1)] + [(MFMA, P) for P in range(1, MFMA_MAX + 1)]
+
+
 @pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
-@pytest.mark.parametrize("P", range(1, MAXK + 1))
-def test_oracle_parity_square(sfmm, orc, op, P):
+@pytest.mark.parametrize("family,P", SQUARE_CASES)
+def test_oracle_parity_square(sfmm, orc, op, family, P):
     n = 200  # several waves and a partial one
     shift = "m2l" if op == W.M2L else "octant"
     items, src, dst = W.uniform_batch(op, P, P, n, 0xA5A5 + 17 * P + op, shift=shift)
     dst0 = W.random_solids(5, 3, n, P).reshape(-1)
-    print(f"op={op} P={P}", flush=True)
-    got = run_gpu(sfmm, op, items, src, dst0)
+    print(f"op={op} family={family} P={P}", flush=True)
+    got = run_gpu(sfmm, op, items, src, dst0, family=family)
     want = orc.translate(op, items, src, dst0.copy())
-    assert_parity(got, want, P, P, label=f"op={op} P={P}")
+    assert_parity(got, want, P, P, label=f"op={op} family={family} P={P}")
 
 
 def test_repeated_targets_apply_in_order(sfmm, orc):
@@ -93,14 +99,15 @@ def test_repeated_targets_apply_in_order(sfmm, orc):
     assert_parity(got, want, 10, 10, tol=1e-12)
 
 
-def test_zero_shift_and_mixed_lanes(sfmm, orc):
+@pytest.mark.parametrize("family", [LANE, MFMA])
+def test_zero_shift_and_mixed_lanes(sfmm, orc, family):
     for op in (W.M2M, W.L2L):
         items, src, dst = W.uniform_batch(op, 8, 8, 130, 12, shift="octant")
         items["x"][::3] = 0.0
         items["y"][::3] = 0.0
         items["z"][::3] = 0.0
         dst0 = W.random_solids(12, 7, 130, 8).reshape(-1)
-        got = run_gpu(sfmm, op, items, src, dst0)
+        got = run_gpu(sfmm, op, items, src, dst0, family=family)
         want = orc.translate(op, items, src, dst0.copy())
         assert_parity(got, want, 8, 8)
 
