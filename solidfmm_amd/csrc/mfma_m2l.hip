@@ -19,13 +19,14 @@ constexpr auto make_table(std::integer_sequence<int, Ps...>)
 }  // namespace
 
 hipError_t launch_m2l(uint32_t P, const sfmm_item *items, size_t n, const double *src, double *dst,
-                         const double *frag, hipStream_t stream)
+                         const double *table, hipStream_t stream)
 {
     static const auto tab = make_table(std::make_integer_sequence<int, kMfmaMax - kMfmaMin + 1>{});
     if (P < uint32_t(kMfmaMin) || P > uint32_t(kMfmaMax)) return hipErrorInvalidValue;
-    const uint32_t grid = static_cast<uint32_t>((n + 15) / 16);
-    hipLaunchKernelGGL(tab.f[P - kMfmaMin], dim3(grid), dim3(64), 0, stream, src, dst, items,
-                       static_cast<uint32_t>(n), frag);
+    const fn_t f = tab.f[P - kMfmaMin];
+    const size_t groups = (n + 15) / 16;
+    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(f), 64 * kWaves, (groups + kWaves - 1) / kWaves);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(64 * kWaves), 0, stream, src, dst, items, static_cast<uint32_t>(n), table);
     return hipGetLastError();
 }
 

@@ -1,4 +1,4 @@
-// Host construction of the MFMA operand table (mfma_tables.hpp) from the
+// Host construction of the dense swap-block table (mfma_tables.hpp) from the
 // compile-time folded swap blocks of tables.hpp, and its device copy.
 #include "mfma_tables.hpp"
 
@@ -16,19 +16,14 @@ void build_table(double *out)
 {
     static_for<0, kMaxOrder>([&](auto nc) {
         constexpr int n = decltype(nc)::value;
-        static_for<0, 4>([&](auto kc) {
-            constexpr int kind = decltype(kc)::value;  // 0,1: blocks re/im; 2,3: transposes
-            const int v = kind / 2, p = kind % 2;
-            for (int b = 0; b < row_blocks(n); ++b)
-                for (int c = 0; c < k_chunks(n); ++c) {
-                    double *f = out + size_t(64) * frag_index(v, p, n, b, c);
-                    for (int lane = 0; lane < 64; ++lane) {
-                        const int m = 16 * b + (lane % 16);
-                        const int l = 4 * c + lane / 16;
-                        f[lane] = (m <= n && l <= n) ? folded<n, kind>(m, l) : 0.0;
-                    }
-                }
-        });
+        const int st = dense_stride(n);
+        double *sr = out + dense_base(n);
+        double *si = sr + dense_part(n);
+        for (int m = 0; m <= n; ++m)
+            for (int l = 0; l < st; ++l) {
+                sr[m * st + l] = (l <= n) ? folded<n, 0>(m, l) : 0.0;
+                si[m * st + l] = (l <= n) ? folded<n, 1>(m, l) : 0.0;
+            }
     });
 }
 

@@ -1,11 +1,13 @@
-// Layout of the MFMA operand table of the folded swap blocks.
+// Dense copies of the folded swap blocks for the MFMA kernels.
 //
-// For degree n the folded blocks (operator_data.cpp:47-147) are cut into
-// row blocks b of 16 orders and K-chunks c of 4 orders; fragment (b, c) is
-// the 16x4 A operand of v_mfma_f64_16x16x4f64 in lane order: lane
-// r + 16k holds block entry [m = 16b + r][l = 4c + k] (zero outside
-// m, l <= n).  Variant v = 0 holds the blocks, v = 1 their transposes; part
-// p = 0 the real block, p = 1 the imaginary one.
+// Degree n holds the real block SR_n and then the imaginary block SI_n
+// (operator_data.cpp:47-147, row = output order m, column = input order l),
+// each (n+1) rows at an odd row stride, so the 16 rows one MFMA operand
+// reads fall in distinct LDS banks.  The transposed blocks the L-type
+// pipeline needs are not stored: TR = D^-1 SR^T D and TI = D^-1 SI^T D with
+// D = diag(1, 2, ..., 2), so a transposed swap reads SR^T / SI^T from the
+// same table, doubles its inputs of order l >= 1 and halves its outputs of
+// order m >= 1 (exact in binary floating point).
 #pragma once
 
 #include <cstddef>
@@ -16,25 +18,21 @@ namespace mfma {
 // Degrees 0..kMaxOrder-1 are tabulated.
 inline constexpr int kMaxOrder = 32;
 
-constexpr int row_blocks(int n) { return (n + 16) / 16; }
-constexpr int k_chunks(int n) { return (n + 4) / 4; }
+constexpr int dense_stride(int n) { return (n + 1) | 1; }
+constexpr int dense_part(int n) { return (n + 1) * dense_stride(n); }
 
-constexpr int degree_base(int n)
+// Offset (in doubles) of degree n's SR block; SI follows at + dense_part(n).
+constexpr int dense_base(int n)
 {
     int s = 0;
-    for (int j = 0; j < n; ++j) s += row_blocks(j) * k_chunks(j);
+    for (int j = 0; j < n; ++j) s += 2 * dense_part(j);
     return s;
 }
 
-constexpr int fragments_per_part() { return degree_base(kMaxOrder); }
+// Doubles needed for degrees 0..P-1.
+constexpr int dense_size(int P) { return dense_base(P); }
 
-// Index (in fragments of 64 doubles) of fragment (v, p, n, b, c).
-constexpr int frag_index(int v, int p, int n, int b, int c)
-{
-    return (v * 2 + p) * fragments_per_part() + degree_base(n) + b * k_chunks(n) + c;
-}
-
-constexpr size_t table_doubles() { return size_t(4) * fragments_per_part() * 64; }
+constexpr size_t table_doubles() { return size_t(dense_size(kMaxOrder)); }
 
 // Fill `out` (table_doubles() reals) on the host.
 void build_table(double *out);

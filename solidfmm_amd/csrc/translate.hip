@@ -22,6 +22,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <utility>
+#include <vector>
 
 #include "../../include/solidfmm_gpu.h"
 #include "mfma_tables.hpp"
@@ -530,6 +533,35 @@ bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family)
 }
 
 size_t max_kernel_order() { return size_t(mfma::kMfmaMax); }
+
+uint32_t persistent_grid(const void *kernel, int block, size_t work)
+{
+    struct Key {
+        const void *k;
+        int dev;
+    };
+    static std::mutex mu;
+    static std::vector<std::pair<Key, uint32_t>> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    uint32_t resident = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (auto &e : cache)
+            if (e.first.k == kernel && e.first.dev == dev) resident = e.second;
+        if (!resident) {
+            int per_cu = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+                cus = 1;
+            resident = uint32_t(per_cu) * uint32_t(cus);
+            cache.push_back({{kernel, dev}, resident});
+        }
+    }
+    if (work < resident) return uint32_t(work < 1 ? 1 : work);
+    return resident;
+}
 
 hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_item *items, size_t n,
                             const double *src, double *dst, hipStream_t stream, int family)

@@ -16,6 +16,10 @@ bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family = 0);
 // Highest order with a kernel.
 size_t max_kernel_order();
 
+// Workgroups for a persistent kernel: as many as can be resident on the
+// current device at once, but no more than `work` (one per unit of work).
+uint32_t persistent_grid(const void *kernel, int block, size_t work);
+
 // Kernel families: one lane per translation (FP64 VALU, orders up to 12)
 // and one wave per 16 translations (swaps on the FP64 matrix cores).
 enum Family { kAuto = 0, kLaneKernels = 1, kMfmaKernels = 2 };

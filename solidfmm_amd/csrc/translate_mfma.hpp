@@ -1,6 +1,6 @@
 // FP64 translation kernels whose x<->z swap contractions run on the matrix
-// cores (v_mfma_f64_16x16x4f64), for orders too large for the one-lane-per-
-// translation kernels of translate.hip.
+// cores (v_mfma_f64_16x16x4f64).  Used for the orders the one-lane-per-
+// translation kernels of translate.hip cannot hold in registers.
 //
 // Mapping.  A wave owns 16 translations; lane l = t + 16 q handles
 // translation t and the orders m = q + 4 s ("slot" s) of every degree.  That
@@ -9,10 +9,13 @@
 // wants its B operand (rows 4c..4c+3 of K-chunk c sit in element c of the
 // four quarter-waves).  So a swap is
 //     D[m][t] = sum_c  A_c[m][4c..4c+3] * x[4c..4c+3][t]
-// with x taken straight from the previous result registers: no LDS, no
-// shuffles between the rotation, swap, rotation, swap steps.  The constant
-// A fragments (folded swap blocks, operator_data.cpp:47-147, in MFMA operand
-// order) come from a device table built once per process (mfma_tables.cpp).
+// with x taken straight from the previous result registers: no shuffles
+// between the rotation, swap, rotation, swap steps.
+//
+// The A operands (the folded swap blocks, operator_data.cpp:47-147) are
+// constant; each workgroup copies the dense table of mfma_tables.hpp for
+// its degrees into LDS once and then loops over groups of 16 translations
+// (persistent workgroups), so every operand read is an LDS read.
 //
 // The rotations e^{+-i m theta}, the z-translation sums and the scaling run
 // on the FP64 VALU on the same registers; each lane keeps the rotated
@@ -35,6 +38,8 @@ namespace mfma {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
+inline constexpr int kWaves = 4;  // waves per workgroup
+
 // Triangle storage per lane: slot s holds degrees K in [4s, PI).
 template <int PI>
 constexpr int slot_off(int s)
@@ -55,29 +60,51 @@ constexpr int nslots()
     return (P + 3) / 4;
 }
 
+__device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
+
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c)
 {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// One swap step for degree N on the lane's slot values (xr, xi): both
-// folded blocks (real and imaginary parts) of variant V (0: the blocks,
-// 1: their transposes).
-template <int N, int V, int NSL>
-__device__ __forceinline__ void swap_mfma(const double *__restrict__ frag, int lane, double (&xr)[NSL],
-                                          double (&xi)[NSL])
+// One swap step for degree N on the lane's slot values (xr, xi), with the
+// folded real and imaginary blocks (TR = false) or their transposes
+// (TR = true, via D^-1 S^T D).  tab points at the LDS copy of the table.
+template <int N, bool TR, int NSL>
+__device__ __forceinline__ void swap_mfma(const double *tab, int lane, double (&xr)[NSL], double (&xi)[NSL])
 {
     constexpr int NB = (N + 16) / 16;  // row blocks of 16 orders
     constexpr int NC = (N + 4) / 4;    // K-chunks of 4 orders
+    constexpr int ST = dense_stride(N);
+    const double *sr = tab + dense_base(N);
+    const double *si = sr + dense_part(N);
+    const int r = lane & 15;  // operand row within the block (output order)
+    const int k = lane >> 4;  // operand column within the chunk (input order); also q
+
+    if constexpr (TR) {
+        // inputs of order l = 4c + q >= 1 doubled
+        static_for<0, NC>([&](auto cc) {
+            constexpr int c = decltype(cc)::value;
+            if (c > 0 || k > 0) {
+                xr[c] *= 2.0;
+                xi[c] *= 2.0;
+            }
+        });
+    }
     dbl4 dr[NB], di[NB];
     static_for<0, NB>([&](auto bc) {
         constexpr int b = decltype(bc)::value;
         dr[b] = dbl4{0.0, 0.0, 0.0, 0.0};
         di[b] = dbl4{0.0, 0.0, 0.0, 0.0};
+        const int m = 16 * b + r;
         static_for<0, NC>([&](auto cc) {
             constexpr int c = decltype(cc)::value;
-            const double ar = frag[64 * frag_index(V, 0, N, b, c) + lane];
-            const double ai = frag[64 * frag_index(V, 1, N, b, c) + lane];
+            const int l = 4 * c + k;
+            const bool in = (m <= N) && (l <= N);
+            const int idx = TR ? (l * ST + m) : (m * ST + l);
+            const int safe = in ? idx : 0;
+            const double ar = in ? sr[safe] : 0.0;
+            const double ai = in ? si[safe] : 0.0;
             dr[b] = mfma(ar, xr[c], dr[b]);
             di[b] = mfma(ai, xi[c], di[b]);
         });
@@ -92,6 +119,18 @@ __device__ __forceinline__ void swap_mfma(const double *__restrict__ frag, int l
             }
         });
     });
+    if constexpr (TR) {
+        // outputs of order m = q + 4s + 16b >= 1 halved
+        static_for<0, 4 * NB>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if constexpr (s < NSL) {
+                if (s > 0 || k > 0) {
+                    xr[s] *= 0.5;
+                    xi[s] *= 0.5;
+                }
+            }
+        });
+    }
 }
 
 // Per-lane rotation factors e^{i m alpha}, e^{i m beta} for m = q + 4s.
@@ -100,17 +139,16 @@ struct LaneRot {
     double ca[NSL], sa[NSL], cb[NSL], sb[NSL];
 };
 
-template <int N, int NSL>
-__device__ __forceinline__ void rot(const LaneRot<NSL> &R, bool beta, bool forward, double scale, double (&xr)[NSL],
-                                    double (&xi)[NSL])
+template <int N, bool BETA, bool FWD, int NSL>
+__device__ __forceinline__ void rot(const LaneRot<NSL> &R, double scale, double (&xr)[NSL], double (&xi)[NSL])
 {
     constexpr int NS = (N + 4) / 4;
     static_for<0, NS>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
-        const double c = beta ? R.cb[s] : R.ca[s];
-        const double sn = beta ? R.sb[s] : R.sa[s];
+        const double c = BETA ? R.cb[s] : R.ca[s];
+        const double sn = BETA ? R.sb[s] : R.sa[s];
         const double a = xr[s], b = xi[s];
-        if (forward) {
+        if constexpr (FWD) {
             xr[s] = scale * (c * a - sn * b);
             xi[s] = scale * (sn * a + c * b);
         } else {
@@ -120,38 +158,38 @@ __device__ __forceinline__ void rot(const LaneRot<NSL> &R, bool beta, bool forwa
     });
 }
 
-template <int N, int V, int NSL>
-__device__ __forceinline__ void transform_fwd(const double *frag, int lane, const LaneRot<NSL> &R, double scale,
+template <int N, bool TR, int NSL>
+__device__ __forceinline__ void transform_fwd(const double *tab, int lane, const LaneRot<NSL> &R, double scale,
                                               double (&xr)[NSL], double (&xi)[NSL])
 {
-    rot<N>(R, false, true, scale, xr, xi);
-    swap_mfma<N, V>(frag, lane, xr, xi);
-    rot<N>(R, true, true, 1.0, xr, xi);
-    swap_mfma<N, V>(frag, lane, xr, xi);
+    rot<N, false, true>(R, scale, xr, xi);
+    swap_mfma<N, TR>(tab, lane, xr, xi);
+    rot<N, true, true>(R, 1.0, xr, xi);
+    swap_mfma<N, TR>(tab, lane, xr, xi);
 }
 
-template <int N, int V, int NSL>
-__device__ __forceinline__ void transform_bwd(const double *frag, int lane, const LaneRot<NSL> &R, double scale,
+template <int N, bool TR, int NSL>
+__device__ __forceinline__ void transform_bwd(const double *tab, int lane, const LaneRot<NSL> &R, double scale,
                                               double (&xr)[NSL], double (&xi)[NSL])
 {
-    swap_mfma<N, V>(frag, lane, xr, xi);
-    rot<N>(R, true, false, 1.0, xr, xi);
-    swap_mfma<N, V>(frag, lane, xr, xi);
-    rot<N>(R, false, false, scale, xr, xi);
+    swap_mfma<N, TR>(tab, lane, xr, xi);
+    rot<N, true, false>(R, 1.0, xr, xi);
+    swap_mfma<N, TR>(tab, lane, xr, xi);
+    rot<N, false, false>(R, scale, xr, xi);
 }
 
+// Translations of one group of 16 (items [16g, 16g+16) of the launch).
 template <int OP, int PI, int PO>
-__global__ void __launch_bounds__(64)
-    k_translate_mfma(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
-                     const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ frag)
+__device__ __forceinline__ void translate_group(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
+                                                const sfmm_item *__restrict__ items, uint32_t n_items, uint32_t g,
+                                                const double *tab, int lane)
 {
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int PE = PI < PO ? PI : PO;
     constexpr int NSL = 4 * ((PM + 15) / 16);  // slots incl. padding to whole row blocks
-    const int lane = threadIdx.x;
     const int q = lane >> 4;
     const int t = lane & 15;
-    const uint32_t idx = blockIdx.x * 16u + static_cast<uint32_t>(t);
+    const uint32_t idx = g * 16u + static_cast<uint32_t>(t);
     const bool valid = idx < n_items;
 
     // Every lane stays active to the end: the MFMAs need the whole wave.
@@ -163,7 +201,7 @@ __global__ void __launch_bounds__(64)
 
     bool zero_shift = false;
     if constexpr (OP != SFMM_M2L) zero_shift = valid && (x * x + y * y + z * z == 0.0);
-    if constexpr (OP == SFMM_M2M) {
+    if constexpr (OP == SFMM_M2M) {  // translations.cpp:331-333
         x = -x;
         y = -y;
         z = -z;
@@ -173,7 +211,9 @@ __global__ void __launch_bounds__(64)
         y = 0.0;
         z = 1.0;
     }
+    const bool active = valid && !zero_shift;
 
+    // Euler data (microkernel_generic.cpp:61-74).
     const double rho2 = x * x + y * y;
     const double rho = sqrt(rho2);
     const double r = sqrt(rho2 + z * z);
@@ -220,7 +260,6 @@ __global__ void __launch_bounds__(64)
 
     // Rotated triangle, this lane's orders only.
     double tr[tri_size<PI>()], ti[tri_size<PI>()];
-    const bool active = valid && !zero_shift;
 
     auto load_degree = [&](auto nc, double(&xr)[NSL], double(&xi)[NSL]) {
         constexpr int n = decltype(nc)::value;
@@ -247,27 +286,55 @@ __global__ void __launch_bounds__(64)
             }
         });
     };
-    auto add_degree = [&](auto Nc, const double(&xr)[NSL], const double(&xi)[NSL]) {
+    auto fetch_old = [&](auto Nc, double(&orr)[NSL], double(&oi)[NSL]) {
+        constexpr int N = decltype(Nc)::value;
+        static_for<0, NSL>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            orr[s] = 0.0;
+            oi[s] = 0.0;
+            if constexpr (4 * s <= N) {
+                const int m = q + 4 * s;
+                if (active && m <= N) {
+                    orr[s] = D[N * (N + 1) + m];
+                    oi[s] = D[(N + 1) * (N + 1) + m];
+                }
+            }
+        });
+    };
+    auto store_sum = [&](auto Nc, const double(&orr)[NSL], const double(&oi)[NSL], const double(&xr)[NSL],
+                         const double(&xi)[NSL]) {
         constexpr int N = decltype(Nc)::value;
         static_for<0, NSL>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
             if constexpr (4 * s <= N) {
                 const int m = q + 4 * s;
                 if (active && m <= N) {
-                    D[N * (N + 1) + m] += xr[s];
-                    D[(N + 1) * (N + 1) + m] += xi[s];
+                    D[N * (N + 1) + m] = orr[s] + xr[s];
+                    D[(N + 1) * (N + 1) + m] = oi[s] + xi[s];
                 }
             }
         });
     };
+    // Source and target values are fetched one degree ahead of their use;
+    // the scheduling fences keep each fetch where it is issued.
+    double pr[NSL], pim[NSL];  // prefetched source degree
+    double orr[NSL], oi[NSL];  // prefetched target degree
+    auto take = [](double(&to)[NSL], const double(&from)[NSL]) {
+        static_for<0, NSL>([&](auto sc) { to[decltype(sc)::value] = from[decltype(sc)::value]; });
+    };
 
     if constexpr (OP == SFMM_M2L) {
         double sc = rinv;  // r^-(n+1) (translations.cpp:45)
+        load_degree(ic<0>{}, pr, pim);
         static_for<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             double xr[NSL], xi[NSL];
-            load_degree(nc, xr, xi);
-            transform_fwd<n, 0>(frag, lane, R, sc, xr, xi);
+            take(xr, pr);
+            take(xi, pim);
+            if constexpr (n + 1 < PI) load_degree(ic<n + 1>{}, pr, pim);
+            else fetch_old(ic<PO - 1>{}, orr, oi);
+            sched_fence();
+            transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
             keep_degree(nc, xr, xi);
             sc *= rinv;
         });
@@ -279,6 +346,11 @@ __global__ void __launch_bounds__(64)
         });
         static_for_down<0, PO>([&](auto Nc) {
             constexpr int N = decltype(Nc)::value;
+            double cr[NSL], ci[NSL];
+            take(cr, orr);
+            take(ci, oi);
+            if constexpr (N > 0) fetch_old(ic<N - 1>{}, orr, oi);
+            sched_fence();
             double xr[NSL], xi[NSL];
             const bool odd = ((N + q) & 1) != 0;  // (-1)^(N+m), m = q + 4s
             static_for<0, NSL>([&](auto sc) {
@@ -303,24 +375,35 @@ __global__ void __launch_bounds__(64)
                 xr[s] = ar;
                 xi[s] = ai;
             });
-            transform_bwd<N, 1>(frag, lane, R, rp[N], xr, xi);
-            add_degree(Nc, xr, xi);
+            transform_bwd<N, true>(tab, lane, R, rp[N], xr, xi);
+            store_sum(Nc, cr, ci, xr, xi);
         });
     } else if constexpr (OP == SFMM_M2M) {
         double sc = rinv;  // forward r^-(n+1)
         double sb = r;     // backward r^(N+1) (translations.cpp:129)
+        load_degree(ic<0>{}, pr, pim);
+        fetch_old(ic<0>{}, orr, oi);
         static_for<0, PM>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
+            double xr[NSL], xi[NSL], cr[NSL], ci[NSL];
             if constexpr (n < PI) {
-                double xr[NSL], xi[NSL];
-                load_degree(nc, xr, xi);
-                transform_fwd<n, 0>(frag, lane, R, sc, xr, xi);
+                take(xr, pr);
+                take(xi, pim);
+                if constexpr (n + 1 < PI) load_degree(ic<n + 1>{}, pr, pim);
+            }
+            if constexpr (n < PO) {
+                take(cr, orr);
+                take(ci, oi);
+                if constexpr (n + 1 < PO) fetch_old(ic<n + 1>{}, orr, oi);
+            }
+            sched_fence();
+            if constexpr (n < PI) {
+                transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
                 keep_degree(nc, xr, xi);
                 sc *= rinv;
             }
             if constexpr (n < PO) {
                 constexpr int N = n;
-                double xr[NSL], xi[NSL];
                 static_for<0, NSL>([&](auto scc) {
                     constexpr int s = decltype(scc)::value;
                     double ar = 0.0, ai = 0.0;
@@ -340,13 +423,14 @@ __global__ void __launch_bounds__(64)
                     xr[s] = ar;
                     xi[s] = ai;
                 });
-                transform_bwd<N, 0>(frag, lane, R, sb, xr, xi);
-                add_degree(ic<N>{}, xr, xi);
+                transform_bwd<N, false>(tab, lane, R, sb, xr, xi);
+                store_sum(ic<N>{}, cr, ci, xr, xi);
                 sb *= r;
             }
         });
     } else {
-        // L2L: forward scale r^n, backward r^-n, descending degrees.
+        // L2L: forward scale r^n (translations.cpp:198), backward r^-n
+        // (:180), descending degrees.
         double fs[PI];
         fs[0] = 1.0;
         static_for<1, PI>([&](auto nc) {
@@ -359,17 +443,26 @@ __global__ void __launch_bounds__(64)
             constexpr int n = decltype(nc)::value;
             bs[n] = bs[n - 1] * rinv;
         });
+        load_degree(ic<PI - 1>{}, pr, pim);
+        if constexpr (PE - 1 == PI - 1) fetch_old(ic<PE - 1>{}, orr, oi);
         static_for_down<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
-            {
-                double xr[NSL], xi[NSL];
-                load_degree(nc, xr, xi);
-                transform_fwd<n, 1>(frag, lane, R, fs[n], xr, xi);
-                keep_degree(nc, xr, xi);
+            double xr[NSL], xi[NSL], cr[NSL], ci[NSL];
+            take(xr, pr);
+            take(xi, pim);
+            if constexpr (n > 0) load_degree(ic<n - 1>{}, pr, pim);
+            if constexpr (n < PE) {
+                take(cr, orr);
+                take(ci, oi);
+                if constexpr (n > 0) fetch_old(ic<n - 1>{}, orr, oi);
+            } else if constexpr (n == PE) {
+                fetch_old(ic<PE - 1>{}, orr, oi);
             }
+            sched_fence();
+            transform_fwd<n, true>(tab, lane, R, fs[n], xr, xi);
+            keep_degree(nc, xr, xi);
             if constexpr (n < PE) {
                 constexpr int N = n;
-                double xr[NSL], xi[NSL];
                 static_for<0, NSL>([&](auto scc) {
                     constexpr int s = decltype(scc)::value;
                     double ar = 0.0, ai = 0.0;
@@ -388,8 +481,8 @@ __global__ void __launch_bounds__(64)
                     xr[s] = ar;
                     xi[s] = ai;
                 });
-                transform_bwd<N, 1>(frag, lane, R, bs[N], xr, xi);
-                add_degree(ic<N>{}, xr, xi);
+                transform_bwd<N, true>(tab, lane, R, bs[N], xr, xi);
+                store_sum(ic<N>{}, cr, ci, xr, xi);
             }
         });
     }
@@ -401,6 +494,26 @@ __global__ void __launch_bounds__(64)
         constexpr int len = PE * (PE + 1);
         for (int i = q; i < len; i += 4) D[i] += S[i];
     }
+}
+
+// Persistent workgroups of kWaves waves: stage the swap blocks of degrees
+// < max(PI, PO) in LDS, then each wave takes groups of 16 translations.
+template <int OP, int PI, int PO>
+__global__ void __launch_bounds__(64 * kWaves)
+    k_translate_mfma(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
+                     const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ table)
+{
+    constexpr int PM = PI > PO ? PI : PO;
+    constexpr int TAB = dense_size(PM);
+    __shared__ double tab[TAB];
+    for (int i = threadIdx.x; i < TAB; i += 64 * kWaves) tab[i] = table[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t groups = (n_items + 15u) / 16u;
+    for (uint32_t g = blockIdx.x * kWaves + wave; g < groups; g += gridDim.x * kWaves)
+        translate_group<OP, PI, PO>(src_pool, dst_pool, items, n_items, g, tab, lane);
 }
 
 }  // namespace mfma
