@@ -52,7 +52,8 @@ enum {
 enum {
     SFMM_KERNELS_AUTO = 0, /* per order: the faster family that has a kernel */
     SFMM_KERNELS_LANE = 1, /* one lane per translation, FP64 VALU (orders <= 12) */
-    SFMM_KERNELS_MFMA = 2  /* 16 translations per wave, swaps on FP64 MFMA (orders <= 24) */
+    SFMM_KERNELS_MFMA = 2, /* 16 translations per wave, swaps on FP64 MFMA (orders <= 24) */
+    SFMM_KERNELS_GENERIC = 3 /* one workgroup per translation, any orders <= 86 */
 };
 
 /* Error codes (mirror the reference's exception types). */
@@ -80,7 +81,7 @@ typedef struct sfmm_item {
 typedef struct sfmm_context sfmm_context;
 typedef struct sfmm_plan sfmm_plan;
 
-/* Highest order this build supports (FP64). */
+/* Highest order this build supports (FP64): 86, the reference's limit. */
 size_t sfmm_max_order(void);
 
 /* Context on HIP device `device` for orders up to P.  Replaces
@@ -128,6 +129,11 @@ int sfmm_launch(sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, const 
  * classic single-thread entry points use this path. */
 int sfmm_translate_host(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, const double *src_pool,
                         size_t src_len, double *dst_pool, size_t dst_len);
+
+/* Diagnostics: the folded swap block of degree n (part 0: real, 1:
+ * imaginary; operator_data.cpp:47-147) as the kernels use it, row-major
+ * (n+1) x (n+1) into out.  Host only; needs no device. */
+int sfmm_swap_block(int n, int part, double *out);
 
 /* Blocks until all work the context issued on `stream` is done. */
 int sfmm_synchronize(sfmm_context *ctx, void *stream);

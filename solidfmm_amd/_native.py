@@ -23,7 +23,16 @@ SFMM_ERR_UNSUPPORTED = -6
 
 SFMM_FP64 = 0
 
-KERNELS_AUTO, KERNELS_LANE, KERNELS_MFMA = 0, 1, 2
+KERNELS_AUTO, KERNELS_LANE, KERNELS_MFMA, KERNELS_GENERIC = 0, 1, 2, 3
+
+
+def swap_block(n, part):
+    """The folded swap block of degree n (part 0 real, 1 imaginary) as the
+    kernels use it; host only."""
+    out = np.zeros((n + 1) * (n + 1))
+    rc = load().sfmm_swap_block(n, part, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    _check(rc, None, "sfmm_swap_block")
+    return out.reshape(n + 1, n + 1)
 
 # Every function include/solidfmm_gpu.h declares, with its ctypes signature.
 _c = ctypes
@@ -42,6 +51,7 @@ SIGNATURES = {
     "sfmm_plan_execute": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
     "sfmm_launch": (_c.c_int, [_vp, _c.c_int, _c.c_uint32, _c.c_uint32, _vp, _c.c_size_t, _vp, _vp, _vp]),
     "sfmm_translate_host": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t]),
+    "sfmm_swap_block": (_c.c_int, [_c.c_int, _c.c_int, _c.POINTER(_c.c_double)]),
     "sfmm_synchronize": (_c.c_int, [_vp, _vp]),
 }
 

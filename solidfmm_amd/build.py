@@ -26,13 +26,22 @@ COMMON = [
 
 # Translation units; the kernel instantiations are split so they compile in
 # parallel.
-SOURCES = ["translate.hip", "mfma_m2m.hip", "mfma_m2l.hip", "mfma_l2l.hip", "mfma_tables.cpp", "capi.cpp"]
+SOURCES = ["translate.hip", "mfma_m2m.hip", "mfma_m2l.hip", "mfma_l2l.hip", "swap_host.cpp", "generic.hip", "capi.cpp",
+           "dropin.cpp"]
+
+# Host-only C++ (the drop-in API over the C ABI).
+HOST_ONLY = {"dropin.cpp"}
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
 def _compile(src):
     obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
-    lang = [] if src.endswith(".hip") else ["-x", "hip"]
-    cmd = [HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if src in HOST_ONLY:
+        cmd = [HIPCC, "-O3", "-std=c++20", "-fPIC", "-Wall", "-x", "c++", "-I", INCLUDE,
+               "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        cmd = [HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -141,7 +141,7 @@ const char *sfmm_last_error(const sfmm_context *ctx) { return ctx ? ctx->err.c_s
 int sfmm_set_kernels(sfmm_context *ctx, int family)
 {
     if (!ctx) return SFMM_ERR_LOGIC;
-    if (family < SFMM_KERNELS_AUTO || family > SFMM_KERNELS_MFMA)
+    if (family < SFMM_KERNELS_AUTO || family > SFMM_KERNELS_GENERIC)
         return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown kernel family");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->family = family;

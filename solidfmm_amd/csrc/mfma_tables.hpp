@@ -15,8 +15,9 @@
 namespace sfmm {
 namespace mfma {
 
-// Degrees 0..kMaxOrder-1 are tabulated.
-inline constexpr int kMaxOrder = 32;
+// Degrees 0..kMaxOrder-1 are tabulated (the FP64 limit of the reference,
+// operator_data.cpp:35-45, is order 86).
+inline constexpr int kMaxOrder = 86;
 
 constexpr int dense_stride(int n) { return (n + 1) | 1; }
 constexpr int dense_part(int n) { return (n + 1) * dense_stride(n); }
@@ -34,12 +35,16 @@ constexpr int dense_size(int P) { return dense_base(P); }
 
 constexpr size_t table_doubles() { return size_t(dense_size(kMaxOrder)); }
 
-// Fill `out` (table_doubles() reals) on the host.
+// Fill `out` (table_doubles() reals) on the host (swap_host.cpp).
 void build_table(double *out);
 
 // Device copy of the table on the current device, built on first use and
 // kept until the process ends.  Returns nullptr on failure.
 const double *device_table();
+
+// Device factorials n! for n < 2 kMaxOrder, followed by 1/n! for
+// n < 2 kMaxOrder (operator_data.cpp:195-214 arithmetic).
+const double *device_factorials();
 
 }  // namespace mfma
 }  // namespace sfmm

@@ -524,15 +524,16 @@ constexpr uint32_t kLaneAutoMax = 10;
 
 bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family)
 {
-    if (op < 0 || op > 2 || p_in != p_out || p_in < 1) return false;
+    if (op < 0 || op > 2 || p_in < 1 || p_out < 1) return false;
+    const bool square = p_in == p_out;
     switch (family) {
-    case kLaneKernels: return p_in <= uint32_t(kSquare);
-    case kMfmaKernels: return p_in >= uint32_t(mfma::kMfmaMin) && p_in <= uint32_t(mfma::kMfmaMax);
-    default: return p_in <= uint32_t(kSquare) || p_in <= uint32_t(mfma::kMfmaMax);
+    case kLaneKernels: return square && p_in <= uint32_t(kSquare);
+    case kMfmaKernels: return square && p_in >= uint32_t(mfma::kMfmaMin) && p_in <= uint32_t(mfma::kMfmaMax);
+    default: return p_in <= uint32_t(mfma::kMaxOrder) && p_out <= uint32_t(mfma::kMaxOrder);
     }
 }
 
-size_t max_kernel_order() { return size_t(mfma::kMfmaMax); }
+size_t max_kernel_order() { return size_t(mfma::kMaxOrder); }
 
 uint32_t persistent_grid(const void *kernel, int block, size_t work)
 {
@@ -569,7 +570,11 @@ hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_it
     if (!have_kernel(op, p_in, p_out, family)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
-    const bool lane = family == kLaneKernels || (family == kAuto && p_in <= kLaneAutoMax);
+    const bool square = p_in == p_out;
+    const bool lane = family == kLaneKernels ||
+                      (family == kAuto && square && p_in <= kLaneAutoMax);
+    const bool mfma_k = !lane && (family == kMfmaKernels ||
+                                  (family == kAuto && square && p_in <= uint32_t(mfma::kMfmaMax)));
     if (lane) {
         kernel_fn f = square_table().f[op][p_in - 1];
         const uint32_t block = 64;
@@ -577,13 +582,16 @@ hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_it
         hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n));
         return hipGetLastError();
     }
-    const double *frag = mfma::device_table();
-    if (!frag) return hipErrorOutOfMemory;
-    switch (op) {
-    case SFMM_M2M: return mfma::launch_m2m(p_in, items, n, src, dst, frag, stream);
-    case SFMM_M2L: return mfma::launch_m2l(p_in, items, n, src, dst, frag, stream);
-    default: return mfma::launch_l2l(p_in, items, n, src, dst, frag, stream);
+    if (mfma_k) {
+        const double *table = mfma::device_table();
+        if (!table) return hipErrorOutOfMemory;
+        switch (op) {
+        case SFMM_M2M: return mfma::launch_m2m(p_in, items, n, src, dst, table, stream);
+        case SFMM_M2L: return mfma::launch_m2l(p_in, items, n, src, dst, table, stream);
+        default: return mfma::launch_l2l(p_in, items, n, src, dst, table, stream);
+        }
     }
+    return launch_generic(op, p_in > p_out ? p_in : p_out, items, n, src, dst, stream);
 }
 
 }  // namespace sfmm

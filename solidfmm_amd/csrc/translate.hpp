@@ -22,7 +22,8 @@ uint32_t persistent_grid(const void *kernel, int block, size_t work);
 
 // Kernel families: one lane per translation (FP64 VALU, orders up to 12)
 // and one wave per 16 translations (swaps on the FP64 matrix cores).
-enum Family { kAuto = 0, kLaneKernels = 1, kMfmaKernels = 2 };
+// and the generic kernel (one workgroup per translation, any orders).
+enum Family { kAuto = 0, kLaneKernels = 1, kMfmaKernels = 2, kGenericKernel = 3 };
 
 namespace mfma {
 inline constexpr int kMfmaMin = 1;
@@ -31,6 +32,10 @@ hipError_t launch_m2m(uint32_t P, const sfmm_item *, size_t, const double *, dou
 hipError_t launch_m2l(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
 hipError_t launch_l2l(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
 }  // namespace mfma
+
+// The generic kernel: items of any orders up to `pc`.
+hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
+                          hipStream_t stream);
 
 // Enqueue dst += T(src) for n device-resident items, all of orders
 // (p_in, p_out) and with pairwise distinct targets.

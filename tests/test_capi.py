@@ -83,3 +83,12 @@ def test_workload_prng_is_stable():
         z = z ^ (z >> np.uint64(31))
     assert u[0] == float(z >> np.uint64(11)) * 2.0 ** -53
     assert np.all((u >= 0) & (u < 1))
+
+
+def test_swap_blocks_equal_oracle(lib, orc):
+    """The library's swap blocks (runtime recurrence, every degree the kernels
+    read) equal the oracle's bit for bit; the oracle's are pinned by B_4 and
+    by the compiled reference."""
+    for n in list(range(0, 40)) + [50, 63, 70, 85]:
+        for part in (0, 1):
+            assert np.array_equal(_native.swap_block(n, part), orc.folded(part, n)), (n, part)

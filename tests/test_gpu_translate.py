@@ -51,12 +51,12 @@ def run_gpu(sfmm, op, items, src, dst0, P=None, family=0):
 GOLDEN = load_golden()
 LANE_MAX = 12   # one lane per translation
 MFMA_MAX = 24   # 16 translations per wave, swaps on the matrix cores
-LANE, MFMA = 1, 2
+AUTO, LANE, MFMA, GENERIC = 0, 1, 2, 3
 
 
 def _kernel_available(items):
     pi, po = orders(items)
-    return pi == po and pi <= MFMA_MAX
+    return max(pi, po) <= 86
 
 
 @pytest.mark.parametrize("case", sorted(GOLDEN))
@@ -70,7 +70,40 @@ def test_golden(sfmm, case):
     assert_parity(got, dst, pi, po, label=case)
 
 
-SQUARE_CASES = [(LANE, P) for P in range(1, LANE_MAX + 1)] + [(MFMA, P) for P in range(1, MFMA_MAX + 1)]
+SQUARE_CASES = ([(LANE, P) for P in range(1, LANE_MAX + 1)] + [(MFMA, P) for P in range(1, MFMA_MAX + 1)]
+                + [(GENERIC, P) for P in (1, 2, 3, 5, 10, 16, 24, 30)])
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN))
+def test_golden_generic(sfmm, case):
+    """Every golden case through the generic kernel as well."""
+    op, items, src, dst0, dst = GOLDEN[case]
+    got = run_gpu(sfmm, op, items, src, dst0, family=GENERIC)
+    pi, po = orders(items)
+    assert_parity(got, dst, pi, po, label=case)
+
+
+@pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
+@pytest.mark.parametrize("pi,po", [(4, 20), (20, 4), (13, 7), (7, 13), (2, 17), (17, 16), (40, 40), (60, 30),
+                                   (86, 86)])
+def test_generic_mixed_and_high_orders(sfmm, orc, op, pi, po):
+    """Orders the specialised kernels do not cover, up to the FP64 limit 86.
+    L2L keeps p_in <= p_out + 1 (the reference's own range, see the oracle)."""
+    if op == W.L2L and pi > po + 1:
+        pytest.skip("l2l truncation covered by the oracle against the naive formula")
+    shift = "m2l" if op == W.M2L else "octant"
+    n = 40 if max(pi, po) <= 30 else 6
+    items, src, dst = W.uniform_batch(op, pi, po, n, 0xBEEF + 7 * pi + po + op, shift=shift)
+    if op == W.M2L and max(pi, po) > 30:
+        # keep the expansions convergent and the factorials finite: larger
+        # shifts for high orders
+        for c in ("x", "y", "z"):
+            items[c] *= 4.0
+    dst0 = W.random_solids(9, 4, n, po).reshape(-1)
+    got = run_gpu(sfmm, op, items, src, dst0, family=AUTO)
+    want = orc.translate(op, items, src, dst0.copy())
+    tol = 1e-9 if max(pi, po) <= 30 else 1e-6
+    assert_parity(got, want, pi, po, tol=tol, label=f"{op} {pi}->{po}")
 
 
 @pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
