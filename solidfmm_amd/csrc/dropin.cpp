@@ -15,7 +15,9 @@
 // reference queues them (translations.cpp:654-658).
 #include <solidfmm/handles.hpp>
 #include <solidfmm/harmonics.hpp>
+#include <solidfmm/operator_data.hpp>
 #include <solidfmm/solid.hpp>
+#include <solidfmm/threadlocal_buffer.hpp>
 #include <solidfmm/translations.hpp>
 
 #include <algorithm>
@@ -299,52 +301,95 @@ template <> constexpr size_t type_limit<float >() { return 18; }  // (2P-2)! mus
 template <> constexpr size_t type_limit<double>() { return 86; }
 
 template <typename real>
-class operator_data
+operator_data<real>::operator_data( size_t arg_P ): P { arg_P }, ctx { nullptr }
 {
-public:
-    explicit operator_data( size_t arg_P ): P { arg_P }
-    {
-        if (P == 0) throw std::logic_error { "solidfmm::operator_data(): Requested order P = 0." };
-        if (P > type_limit<real>() || P > sfmm_max_order())
-            throw std::overflow_error { "solidfmm::operator_data(): "
-                                        "Requested order is too large for the chosen floating point type." };
-        int rc = sfmm_create(&ctx, 0, P, SFMM_FP64);
-        if (rc == SFMM_ERR_ALLOC) throw std::bad_alloc {};
-        if (rc != SFMM_OK) throw std::runtime_error { "solidfmm::operator_data(): no usable GPU (HIP error)" };
-    }
-    operator_data( const operator_data & ) = delete;
-    operator_data &operator=( const operator_data & ) = delete;
-    ~operator_data() { sfmm_destroy(ctx); }
+    if (P == 0) throw std::logic_error { "solidfmm::operator_data(): Requested order P = 0." };
+    if (P > type_limit<real>() || P > sfmm_max_order())
+        throw std::overflow_error { "solidfmm::operator_data(): "
+                                    "Requested order is too large for the chosen floating point type." };
+    int rc = sfmm_create(&ctx, 0, P, SFMM_FP64);
+    if (rc == SFMM_ERR_ALLOC) throw std::bad_alloc {};
+    if (rc != SFMM_OK) throw std::runtime_error { "solidfmm::operator_data(): no usable GPU (HIP error)" };
+}
 
-    size_t order() const noexcept { return P; }
-    sfmm_context *context() const noexcept { return ctx; }
+template <typename real>
+operator_data<real>::~operator_data() { sfmm_destroy(ctx); }
 
-private:
-    size_t P;
-    sfmm_context *ctx = nullptr;
-};
+template class operator_data<float>;
+template class operator_data<double>;
 
 // Host staging reused across calls: the packed pools and the item list.
 template <typename real>
-class threadlocal_buffer
-{
-public:
-    threadlocal_buffer() noexcept = default;
-    explicit threadlocal_buffer( const operator_data<real> &op ): P { op.order() } {}
-    threadlocal_buffer( const threadlocal_buffer &rhs ): P { rhs.P } {}
-    threadlocal_buffer &operator=( const threadlocal_buffer &rhs )
-    {
-        P = rhs.P;
-        return *this;
-    }
-    bool compatible( const operator_data<real> &op ) const noexcept { return P >= op.order(); }
-
-    size_t P = 0;
+struct threadlocal_buffer<real>::staging {
     std::vector<double> src, dst;
     std::vector<sfmm_item> items;
     std::unordered_map<const void *, uint64_t> src_off, dst_off;
     std::vector<std::pair<solid<real> *, uint64_t>> targets;
 };
+
+template <typename real>
+threadlocal_buffer<real>::threadlocal_buffer() noexcept {}
+
+template <typename real>
+threadlocal_buffer<real>::threadlocal_buffer( const operator_data<real> &op ): P { op.order() }, s { new staging }
+{
+}
+
+template <typename real>
+threadlocal_buffer<real>::threadlocal_buffer( const threadlocal_buffer &rhs ):
+    P { rhs.P }, s { rhs.s ? new staging : nullptr }
+{
+}
+
+template <typename real>
+threadlocal_buffer<real>::threadlocal_buffer( threadlocal_buffer &&rhs ) noexcept: P { rhs.P }, s { rhs.s }
+{
+    rhs.P = 0;
+    rhs.s = nullptr;
+}
+
+template <typename real>
+threadlocal_buffer<real> &threadlocal_buffer<real>::operator=( const threadlocal_buffer &rhs )
+{
+    if (this != &rhs) {
+        threadlocal_buffer tmp { rhs };
+        swap(tmp);
+    }
+    return *this;
+}
+
+template <typename real>
+threadlocal_buffer<real> &threadlocal_buffer<real>::operator=( threadlocal_buffer &&rhs ) noexcept
+{
+    swap(rhs);
+    return *this;
+}
+
+template <typename real>
+threadlocal_buffer<real>::~threadlocal_buffer() { delete s; }
+
+template <typename real>
+void threadlocal_buffer<real>::swap( threadlocal_buffer &rhs ) noexcept
+{
+    std::swap(P, rhs.P);
+    std::swap(s, rhs.s);
+}
+
+template <typename real>
+void threadlocal_buffer<real>::initialise( const operator_data<real> &op )
+{
+    threadlocal_buffer tmp { op };
+    swap(tmp);
+}
+
+template <typename real>
+bool threadlocal_buffer<real>::compatible( const operator_data<real> &op ) const noexcept
+{
+    return s != nullptr && P >= op.order();
+}
+
+template class threadlocal_buffer<float>;
+template class threadlocal_buffer<double>;
 
 template <typename real> constexpr size_t default_order();
 template <> constexpr size_t default_order<float >() { return 18; }  // handles.cpp:32-33
@@ -432,31 +477,32 @@ template <typename real>
 int run( int op, const operator_data<real> &od, threadlocal_buffer<real> &buf, const translation_info<real> *b,
          const translation_info<real> *e )
 {
-    buf.src.clear();
-    buf.dst.clear();
-    buf.items.clear();
-    buf.src_off.clear();
-    buf.dst_off.clear();
-    buf.targets.clear();
+    typename threadlocal_buffer<real>::staging &st = *buf.stage();
+    st.src.clear();
+    st.dst.clear();
+    st.items.clear();
+    st.src_off.clear();
+    st.dst_off.clear();
+    st.targets.clear();
     for (const translation_info<real> *i = b; i != e; ++i) {
         const solid<real> *s = i->source;
         solid<real> *t = i->target;
         const size_t pin = s->order(), pout = t->order(), dim = s->dimension();
         if (pin == 0 || pout == 0) continue;
-        auto si = buf.src_off.find(s);
+        auto si = st.src_off.find(s);
         uint64_t so;
-        if (si == buf.src_off.end()) {
-            so = buf.src.size();
-            buf.src_off.emplace(s, so);
-            buf.src.insert(buf.src.end(), s->memptr(), s->memptr() + pin * (pin + 1) * dim);
+        if (si == st.src_off.end()) {
+            so = st.src.size();
+            st.src_off.emplace(s, so);
+            st.src.insert(st.src.end(), s->memptr(), s->memptr() + pin * (pin + 1) * dim);
         } else so = si->second;
-        auto ti = buf.dst_off.find(t);
+        auto ti = st.dst_off.find(t);
         uint64_t to;
-        if (ti == buf.dst_off.end()) {
-            to = buf.dst.size();
-            buf.dst_off.emplace(t, to);
-            buf.dst.insert(buf.dst.end(), t->memptr(), t->memptr() + pout * (pout + 1) * dim);
-            buf.targets.emplace_back(t, to);
+        if (ti == st.dst_off.end()) {
+            to = st.dst.size();
+            st.dst_off.emplace(t, to);
+            st.dst.insert(st.dst.end(), t->memptr(), t->memptr() + pout * (pout + 1) * dim);
+            st.targets.emplace_back(t, to);
         } else to = ti->second;
         for (size_t d = 0; d < dim; ++d) {
             sfmm_item it {};
@@ -467,17 +513,17 @@ int run( int op, const operator_data<real> &od, threadlocal_buffer<real> &buf, c
             it.x = i->x;
             it.y = i->y;
             it.z = i->z;
-            buf.items.push_back(it);
+            st.items.push_back(it);
         }
     }
-    if (buf.items.empty()) return SFMM_OK;
-    int rc = sfmm_translate_host(od.context(), op, buf.items.data(), buf.items.size(), buf.src.data(),
-                                 buf.src.size(), buf.dst.data(), buf.dst.size());
+    if (st.items.empty()) return SFMM_OK;
+    int rc = sfmm_translate_host(od.context(), op, st.items.data(), st.items.size(), st.src.data(),
+                                 st.src.size(), st.dst.data(), st.dst.size());
     if (rc != SFMM_OK) return rc;
-    for (auto &tg : buf.targets) {
+    for (auto &tg : st.targets) {
         solid<real> *t = tg.first;
         const size_t len = t->order() * (t->order() + 1) * t->dimension();
-        for (size_t k = 0; k < len; ++k) t->memptr()[k] = static_cast<real>(buf.dst[tg.second + k]);
+        for (size_t k = 0; k < len; ++k) t->memptr()[k] = static_cast<real>(st.dst[tg.second + k]);
     }
     return SFMM_OK;
 }

@@ -26,8 +26,12 @@ def tol_fp64(p_in, p_out):
         t = 2e-12
     elif P <= 16:
         t = 1e-11
-    else:
+    elif P <= 24:
         t = 2e-9
+    else:
+        # random O(1) coefficients at every degree: the oracle and the naive
+        # formulas already differ by 7e-9 at P=30 and 2e-6 at P=40
+        t = 5e-8 if P <= 30 else 1e-4
     return 10 * t if p_in > p_out else t
 
 
