@@ -108,6 +108,108 @@ def copy_mode(args):
                       "ms_per_launch": ms, "GBps": nbytes / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def mixed_mode(args):
+    """Mixed-order M2L: (P_in, P_out) i.i.d. uniform on {pmin..pmax}, one
+    target each.  The autotuner picks a kernel family per order bucket
+    max(P_in, P_out); the plan (one launch per bucket) is captured once into
+    a HIP graph and each step is one graph replay."""
+    import torch
+
+    import solidfmm_amd
+    from solidfmm_amd import workload as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    solidfmm_amd.load()
+    n = args.n if args.n != 100_000 else 500_000
+    seed = 0x5011DF02
+    items, src, dst = W.mixed_batch(n, seed, args.pmin, args.pmax)
+    ctx = solidfmm_amd.Context(args.pmax)
+    ctx.set_kernels(KERNELS[args.kernels])
+    stream = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    if not args.no_autotune and args.kernels == "auto":
+        ctx.autotune(W.M2L, items, stream)
+    tune_s = time.perf_counter() - t0
+    plan = solidfmm_amd.Plan(ctx, W.M2L, items)
+    s = torch.from_numpy(src).cuda()
+    d = torch.zeros(len(dst), dtype=torch.float64, device="cuda")
+    plan.execute(s, d, stream)
+    torch.cuda.synchronize()
+    gpu_out = d.cpu().numpy()
+    if args.no_graph:
+        step = lambda: plan.execute(s, d, stream)  # noqa: E731
+    else:
+        plan.capture(s, d)
+        step = lambda: plan.replay(stream)  # noqa: E731
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    K = args.steps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(K + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for k in range(K):
+        step()
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = ev[0].elapsed_time(ev[K])
+    lat = []
+    for _ in range(args.latency_reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        step()
+        b.record(stream)
+        b.synchronize()
+        lat.append(a.elapsed_time(b))
+    lat = np.array(lat) if lat else np.zeros(1)
+    ms = total_ms / K
+    value = n / (ms * 1e-3)
+    pin = items["p_in"].astype(np.int64)
+    pout = items["p_out"].astype(np.int64)
+    nbytes = float(np.sum(8 * (pin * (pin + 1) + 2 * pout * (pout + 1)) + 24))
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    launches = plan.launch_info()
+    line = {
+        "metric": f"mixed-order M2L translations/s (orders {args.pmin}..{args.pmax}) FP64 on 1 GPU",
+        "value": value, "unit": "translations/s", "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"M2L FP64, {n} translations, P_in and P_out i.i.d. uniform on "
+                               f"{{{args.pmin}..{args.pmax}}}, |shift| in [2,4], one target each, seeded; "
+                               + ("one HIP graph replay per step" if not args.no_graph else "plan execute per step"),
+                   "batch": n, "pmin": args.pmin, "pmax": args.pmax, "kernels": args.kernels,
+                   "autotuned": not args.no_autotune and args.kernels == "auto", "autotune_s": tune_s,
+                   "launches": [{"family": {1: "lane", 2: "mfma", 3: "generic"}[f], "order": o, "per_item_orders": r,
+                                 "items": c} for f, o, r, c in launches]},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "wall_s": wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s", "frac": achieved / 8000.0,
+                     "traffic": None, "algorithmic_bytes_per_step": nbytes,
+                     "note": "all launches of the step (one per order bucket) together"},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        if oracle.Reference.available():
+            ref = oracle.Reference()
+            out = np.zeros_like(dst)
+            secs = ref.translate(W.M2L, args.pmax, items, src, out, threads=threads, repeats=3, chunk=0)
+            line["cpu_baseline"] = {"value": n / secs, "unit": "translations/s", "cores": threads,
+                                    "kind": "reference",
+                                    "sample": f"vorticle/solidfmm 1.5 single-thread m2l_unchecked on {threads} "
+                                              "std::threads (targets hash-partitioned), whole batch, best of 3"}
+        nchk = 2000
+        idx = np.linspace(0, n - 1, nchk).astype(np.int64)
+        sub = items[idx]
+        want = oracle.Oracle().translate(W.M2L, sub, src, np.zeros_like(dst))
+        e = oracle.mixed_errors(gpu_out, want, sub)
+        line["rel_err_vs_oracle"] = {"p50": float(np.percentile(e, 50)), "p99": float(np.percentile(e, 99)),
+                                     "max": float(e.max()), "sample": nchk}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,9 +222,14 @@ def main():
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["translate", "copy"], default="translate",
-                    help="copy: time a plain device copy of --copy-mib MiB (HBM reference and the "
-                         "calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--mode", choices=["translate", "mixed", "copy"], default="translate",
+                    help="mixed: the mixed-order M2L workload (orders --pmin..--pmax, default 500,000 items) "
+                         "as one captured HIP graph; copy: time a plain device copy of --copy-mib MiB (HBM "
+                         "reference and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--pmin", type=int, default=4)
+    ap.add_argument("--pmax", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024)
     args = ap.parse_args()
 
@@ -132,6 +239,8 @@ def main():
         sys.exit(2)
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode == "mixed":
+        return mixed_mode(args)
 
     import torch
 

@@ -48,9 +48,11 @@ enum {
     SFMM_FP64 = 0 /* FP64 throughout (the reference's double path) */
 };
 
-/* Kernel families (sfmm_set_kernels). */
+/* Kernel families (sfmm_set_kernels, sfmm_set_tuning).  Lane and MFMA
+ * kernels exist per order bound P; lists whose items have other orders
+ * p_in, p_out <= P run their per-item-order variants. */
 enum {
-    SFMM_KERNELS_AUTO = 0, /* per order: the faster family that has a kernel */
+    SFMM_KERNELS_AUTO = 0, /* per order bucket: the autotuned family, else the default */
     SFMM_KERNELS_LANE = 1, /* one lane per translation, FP64 VALU (orders <= 12) */
     SFMM_KERNELS_MFMA = 2, /* 16 translations per wave, swaps on FP64 MFMA (orders <= 24) */
     SFMM_KERNELS_GENERIC = 3 /* one workgroup per translation, any orders <= 86 */
@@ -96,18 +98,27 @@ const char *sfmm_last_error(const sfmm_context *ctx);
  * knob with no counterpart in the reference; results agree to rounding). */
 int sfmm_set_kernels(sfmm_context *ctx, int family);
 
-/* Plan: a validated, device-resident translation list, grouped into
- * launches of equal (p_in, p_out) and of pairwise distinct targets.
- * Replaces the validation and queueing of m2m/m2l/l2l
- * (translations.cpp:634-1001): every item is checked before anything is
- * written (orders <= context order, else SFMM_ERR_RANGE; odd offsets
- * SFMM_ERR_LOGIC).  Items sharing a target are applied in list order. */
+/* Plan: a validated, device-resident translation list.  Replaces the
+ * validation and queueing of m2m/m2l/l2l (translations.cpp:634-1001): every
+ * item is checked before anything is written (orders <= context order,
+ * else SFMM_ERR_RANGE; odd offsets SFMM_ERR_LOGIC).  Items sharing a target
+ * are applied in list order ("rounds": the k-th item of a target runs in
+ * round k).  Within a round, items are bucketed by max(p_in, p_out) and
+ * the family chosen for that bucket (sfmm_set_kernels, else the autotuned
+ * family, else the default); each bucket is one launch, its items sorted by
+ * (p_in, p_out). */
 int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, sfmm_plan **plan);
 void sfmm_plan_destroy(sfmm_plan *plan);
 size_t sfmm_plan_launches(const sfmm_plan *plan);
 
 /* Reals of each pool the plan reads or writes: the largest offset+size. */
 void sfmm_plan_extents(const sfmm_plan *plan, size_t *src_len, size_t *dst_len);
+
+/* Launch i of a plan: kernel family, kernel order bound, whether items
+ * carry their own orders (1) or all have the bound as orders (0), and the
+ * number of items. */
+int sfmm_plan_launch_info(const sfmm_plan *plan, size_t i, int *family, uint32_t *order, int *per_item_orders,
+                          size_t *count);
 
 /* Run a plan: target pool += T(source pool), asynchronously on `stream`.
  * Pools are device pointers of src_len / dst_len reals; a pool shorter than
@@ -116,6 +127,27 @@ void sfmm_plan_extents(const sfmm_plan *plan, size_t *src_len, size_t *dst_len);
  * device-resident pools. */
 int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len,
                       void *stream);
+
+/* Capture the plan's launches on these pools into a HIP graph kept with the
+ * plan (replacing an earlier capture); sfmm_plan_replay then enqueues the
+ * whole plan as one graph launch on any stream.  The pools are bound at
+ * capture: capture again to run on other pools.  No counterpart in the
+ * reference (a launch-overhead tool for lists that are replayed, such as
+ * the same interaction lists every time step). */
+int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len);
+int sfmm_plan_replay(sfmm_plan *plan, void *stream);
+
+/* Autotuner: for each order bucket max(p_in, p_out) present in the list,
+ * time every kernel family that covers it on a sample of the bucket's items
+ * (scratch pools, on `stream`; blocks until done) and keep the fastest for
+ * plans created afterwards on this context.  sfmm_get_tuning reports the
+ * family used per order bucket (index = order, entry 0 unused);
+ * sfmm_set_tuning sets one entry (SFMM_KERNELS_AUTO restores the default).
+ * Plays the role of the reference's schedule_by_measuring_throughput
+ * (translations.hpp:72-73): a measured choice of how to run a list. */
+int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, void *stream);
+int sfmm_get_tuning(sfmm_context *ctx, int op, int *family_by_order, size_t len);
+int sfmm_set_tuning(sfmm_context *ctx, int op, uint32_t order, int family);
 
 /* Direct launch over device-resident items the caller has already grouped:
  * all items have orders (p_in, p_out) and pairwise distinct targets.

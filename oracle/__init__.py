@@ -165,3 +165,14 @@ def degree_errors(got, ref, P):
         r = np.linalg.norm(ref[:, idx], axis=1)
         out[:, n] = np.where(r > 0, d / np.where(r > 0, r, 1.0), d)
     return out
+
+
+def mixed_errors(got, want, items):
+    """Per-item error (max over degrees of the per-degree normwise relative
+    error) for lists of mixed orders; target blocks sit at items["dst"]."""
+    e = np.zeros(len(items))
+    for po in np.unique(items["p_out"]):
+        idx = np.nonzero(items["p_out"] == po)[0]
+        cols = items["dst"][idx].astype(np.int64)[:, None] + np.arange(int(po) * (int(po) + 1))
+        e[idx] = degree_errors(got[cols], want[cols], int(po)).max(axis=1)
+    return e

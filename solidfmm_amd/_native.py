@@ -53,6 +53,13 @@ SIGNATURES = {
     "sfmm_translate_host": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t]),
     "sfmm_swap_block": (_c.c_int, [_c.c_int, _c.c_int, _c.POINTER(_c.c_double)]),
     "sfmm_synchronize": (_c.c_int, [_vp, _vp]),
+    "sfmm_plan_launch_info": (_c.c_int, [_vp, _c.c_size_t, _c.POINTER(_c.c_int), _c.POINTER(_c.c_uint32),
+                                         _c.POINTER(_c.c_int), _c.POINTER(_c.c_size_t)]),
+    "sfmm_plan_capture": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t]),
+    "sfmm_plan_replay": (_c.c_int, [_vp, _vp]),
+    "sfmm_autotune": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp]),
+    "sfmm_get_tuning": (_c.c_int, [_vp, _c.c_int, _c.POINTER(_c.c_int), _c.c_size_t]),
+    "sfmm_set_tuning": (_c.c_int, [_vp, _c.c_int, _c.c_uint32, _c.c_int]),
 }
 
 
@@ -168,8 +175,26 @@ class Context:
         return Plan(self, op, items)
 
     def set_kernels(self, family):
-        """Kernel family for plans made afterwards: KERNELS_AUTO, _LANE or _MFMA."""
+        """Kernel family for plans made afterwards: KERNELS_AUTO, _LANE, _MFMA
+        or _GENERIC."""
         _check(load().sfmm_set_kernels(self._h, family), self._h, "sfmm_set_kernels")
+
+    def autotune(self, op, items, stream=None):
+        """Time every kernel family on each order bucket of ``items`` and keep
+        the fastest for plans made afterwards (blocking)."""
+        items = np.ascontiguousarray(items, dtype=ITEM_DTYPE)
+        rc = load().sfmm_autotune(self._h, op, items.ctypes.data, len(items), _stream_ptr(stream))
+        _check(rc, self._h, "sfmm_autotune")
+
+    def tuning(self, op, upto=None):
+        """Family used per order bucket (index = max(p_in, p_out))."""
+        n = (upto or self.P) + 1
+        out = (ctypes.c_int * n)()
+        _check(load().sfmm_get_tuning(self._h, op, out, n), self._h, "sfmm_get_tuning")
+        return list(out)
+
+    def set_tuning(self, op, order, family):
+        _check(load().sfmm_set_tuning(self._h, op, order, family), self._h, "sfmm_set_tuning")
 
     def launch(self, op, p_in, p_out, dev_items, n, src, dst, stream=None):
         """Unchecked launch over device-resident items (``dev_items`` is a uint8
@@ -222,6 +247,28 @@ class Plan:
         dp, dn = _dev_ptr(dst)
         rc = load().sfmm_plan_execute(self._h, sp, sn, dp, dn, _stream_ptr(stream))
         _check(rc, self.ctx.handle, "sfmm_plan_execute")
+
+    def launch_info(self):
+        """[(family, order bound, per-item orders, items)] per launch."""
+        out = []
+        for i in range(self.launches):
+            f, o, r, c = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_int(), ctypes.c_size_t()
+            rc = load().sfmm_plan_launch_info(self._h, i, ctypes.byref(f), ctypes.byref(o), ctypes.byref(r),
+                                              ctypes.byref(c))
+            _check(rc, self.ctx.handle, "sfmm_plan_launch_info")
+            out.append((f.value, o.value, bool(r.value), c.value))
+        return out
+
+    def capture(self, src, dst):
+        """Capture the plan on these pools into a HIP graph (see replay)."""
+        sp, sn = _dev_ptr(src)
+        dp, dn = _dev_ptr(dst)
+        _check(load().sfmm_plan_capture(self._h, sp, sn, dp, dn), self.ctx.handle, "sfmm_plan_capture")
+        self._graph_pools = (src, dst)  # keep the captured pools alive
+
+    def replay(self, stream=None):
+        """dst += T(src) on the captured pools, as one graph launch."""
+        _check(load().sfmm_plan_replay(self._h, _stream_ptr(stream)), self.ctx.handle, "sfmm_plan_replay")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -26,8 +26,8 @@ COMMON = [
 
 # Translation units; the kernel instantiations are split so they compile in
 # parallel.
-SOURCES = ["translate.hip", "mfma_m2m.hip", "mfma_m2l.hip", "mfma_l2l.hip", "swap_host.cpp", "generic.hip", "capi.cpp",
-           "dropin.cpp"]
+SOURCES = ["mfma_m2l_rt.hip", "mfma_m2l.hip", "mfma_m2m_rt.hip", "mfma_l2l_rt.hip", "mfma_m2m.hip", "mfma_l2l.hip",
+           "lane_rt.hip", "lane_fixed.hip", "translate.hip", "generic.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp"]
 
 # Host-only C++ (the drop-in API over the C ABI).
 HOST_ONLY = {"dropin.cpp"}
@@ -50,7 +50,7 @@ def _compile(src):
 
 def build(verbose=True):
     os.makedirs(OBJ_DIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs

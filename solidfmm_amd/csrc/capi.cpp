@@ -1,5 +1,6 @@
 // C ABI of the translation engine (include/solidfmm_gpu.h): contexts,
-// validated plans, launches and the host-pool convenience path.
+// validated plans, launches, graphs, the autotuner and the host-pool
+// convenience path.
 //
 // Validation follows the reference's checked entry points
 // (translations.cpp:634-1001): every item is checked before any work is
@@ -7,6 +8,14 @@
 // list are applied in list order: item k of a target goes to "round" k, and
 // rounds run one after another on the stream, each round a set of launches
 // whose targets are pairwise distinct.
+//
+// Within a round, items are bucketed by their larger order pb =
+// max(p_in, p_out) and the kernel family chosen for (op, pb): the forced
+// family of the context, else the autotuned one, else the default.  A
+// bucket whose items all have p_in = p_out = pb runs the uniform-order
+// kernel; any other bucket runs the per-item-order (RT) variant of the
+// kernel of order pb.  Items of a bucket are sorted by (p_in, p_out), so a
+// wave mostly holds one order pair and skips the degrees it does not need.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,29 +27,36 @@
 #include <vector>
 
 #include "../../include/solidfmm_gpu.h"
+#include "mfma_tables.hpp"
 #include "translate.hpp"
+
+namespace {
+constexpr int kOrders = sfmm::mfma::kMaxOrder + 1;
+}
 
 struct sfmm_context {
     int device = 0;
     size_t P = 0;
     int precision = SFMM_FP64;
     int family = SFMM_KERNELS_AUTO;
+    int tune[3][kOrders] = {};  // autotuned family per (op, pb); 0 = default
     std::string err;
     std::mutex mu;
 };
 
 struct sfmm_plan {
     struct Launch {
-        uint32_t p_in, p_out;
+        sfmm::KernelChoice k;
         size_t offset, count;
     };
     sfmm_context *ctx = nullptr;
     int op = 0;
-    int family = SFMM_KERNELS_AUTO;
     sfmm_item *d_items = nullptr;
     size_t n_items = 0;
     size_t src_extent = 0, dst_extent = 0;
     std::vector<Launch> launches;
+    // captured graph (sfmm_plan_capture)
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -58,6 +74,13 @@ int hip_fail(sfmm_context *ctx, hipError_t e, const char *what)
 
 const char *op_name(int op) { return op == SFMM_M2M ? "m2m" : op == SFMM_M2L ? "m2l" : "l2l"; }
 
+int family_for(const sfmm_context *ctx, int op, uint32_t pb)
+{
+    if (ctx->family != SFMM_KERNELS_AUTO) return ctx->family;
+    const int t = ctx->tune[op][pb];
+    return t ? t : sfmm::default_family(pb);
+}
+
 // Validation and grouping.  Items with an order of zero contribute nothing
 // (the reference pads such lanes with zeros and writes them to its trash
 // buffer, microkernel_generic.cpp:340-380) and are dropped.
@@ -67,7 +90,9 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
     if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
     if (n && !items) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null item list");
     struct Key {
-        uint32_t round, p_in, p_out;
+        uint32_t round;
+        int family;
+        uint32_t pb, p_in, p_out;
         size_t index;
     };
     std::vector<Key> keys;
@@ -86,11 +111,16 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
             return fail(ctx, SFMM_ERR_UNSUPPORTED,
                         std::string("solidfmm: no kernel for ") + op_name(op) + " with p_in=" +
                             std::to_string(it.p_in) + ", p_out=" + std::to_string(it.p_out));
-        uint32_t round = seen[it.dst]++;
-        keys.push_back({round, it.p_in, it.p_out, i});
+        const uint32_t pb = std::max(it.p_in, it.p_out);
+        const int fam = family_for(ctx, op, pb);
+        const uint32_t round = seen[it.dst]++;
+        // the generic kernel takes every order in one launch
+        keys.push_back({round, fam, fam == sfmm::kGenericKernel ? 0u : pb, it.p_in, it.p_out, i});
     }
     std::stable_sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
         if (a.round != b.round) return a.round < b.round;
+        if (a.family != b.family) return a.family < b.family;
+        if (a.pb != b.pb) return a.pb < b.pb;
         if (a.p_in != b.p_in) return a.p_in < b.p_in;
         return a.p_out < b.p_out;
     });
@@ -99,14 +129,39 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
     launches.clear();
     for (size_t k = 0; k < keys.size(); ++k) {
         const Key &key = keys[k];
-        if (launches.empty() || k == 0 || keys[k - 1].round != key.round || keys[k - 1].p_in != key.p_in ||
-            keys[k - 1].p_out != key.p_out)
-            launches.push_back({key.p_in, key.p_out, ordered.size(), 0});
+        if (k == 0 || keys[k - 1].round != key.round || keys[k - 1].family != key.family ||
+            keys[k - 1].pb != key.pb)
+            launches.push_back({{key.family, key.pb, false}, ordered.size(), 0});
+        sfmm_plan::Launch &L = launches.back();
+        if (key.family == sfmm::kGenericKernel) {
+            L.k.pk = std::max({L.k.pk, key.p_in, key.p_out});
+        } else if (key.p_in != key.pb || key.p_out != key.pb) {
+            L.k.rt = true;
+        }
         ordered.push_back(items[key.index]);
-        launches.back().count++;
+        L.count++;
     }
     return SFMM_OK;
 }
+
+void drop_graph(sfmm_plan *plan)
+{
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    plan->exec = nullptr;
+}
+
+hipError_t enqueue(sfmm_plan *plan, const double *src, double *dst, hipStream_t stream)
+{
+    for (const auto &L : plan->launches) {
+        hipError_t e = sfmm::launch_kernel(plan->op, L.k, plan->d_items + L.offset, L.count, src, dst, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Device tables the kernels read, created before any capture so that no
+// allocation happens inside one.
+bool tables_ready() { return sfmm::mfma::device_table() && sfmm::mfma::device_factorials(); }
 
 }  // namespace
 
@@ -163,7 +218,6 @@ int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *items, size_t n
     if (!plan) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
     plan->ctx = ctx;
     plan->op = op;
-    plan->family = ctx->family;
     plan->n_items = ordered.size();
     plan->launches = std::move(launches);
     for (const sfmm_item &it : ordered) {
@@ -190,6 +244,7 @@ int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *items, size_t n
 void sfmm_plan_destroy(sfmm_plan *plan)
 {
     if (!plan) return;
+    drop_graph(plan);
     if (plan->d_items) (void)hipFree(plan->d_items);
     delete plan;
 }
@@ -200,6 +255,18 @@ void sfmm_plan_extents(const sfmm_plan *plan, size_t *src_len, size_t *dst_len)
 {
     if (src_len) *src_len = plan ? plan->src_extent : 0;
     if (dst_len) *dst_len = plan ? plan->dst_extent : 0;
+}
+
+int sfmm_plan_launch_info(const sfmm_plan *plan, size_t i, int *family, uint32_t *order, int *per_item_orders,
+                          size_t *count)
+{
+    if (!plan || i >= plan->launches.size()) return SFMM_ERR_LOGIC;
+    const auto &L = plan->launches[i];
+    if (family) *family = L.k.family;
+    if (order) *order = L.k.pk;
+    if (per_item_orders) *per_item_orders = L.k.rt ? 1 : 0;
+    if (count) *count = L.count;
+    return SFMM_OK;
 }
 
 int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len,
@@ -213,11 +280,52 @@ int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, d
         return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    for (const auto &L : plan->launches) {
-        e = sfmm::launch_translate(plan->op, L.p_in, L.p_out, plan->d_items + L.offset, L.count, src_pool, dst_pool,
-                                   static_cast<hipStream_t>(stream), plan->family);
-        if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+    e = enqueue(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+    return SFMM_OK;
+}
+
+int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len)
+{
+    if (!plan) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = plan->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (plan->n_items && (!src_pool || !dst_pool)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pool");
+    if (src_len < plan->src_extent || dst_len < plan->dst_extent)
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    if (!tables_ready()) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: device tables");
+    drop_graph(plan);
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate");
+    hipGraph_t graph = nullptr;
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+        hipError_t le = enqueue(plan, src_pool, dst_pool, s);
+        e = hipStreamEndCapture(s, &graph);
+        if (le != hipSuccess) e = le;
     }
+    if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) {
+        plan->exec = nullptr;
+        return hip_fail(ctx, e, "graph capture");
+    }
+    return SFMM_OK;
+}
+
+int sfmm_plan_replay(sfmm_plan *plan, void *stream)
+{
+    if (!plan) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = plan->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (!plan->exec) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan has no captured graph");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipGraphLaunch");
     return SFMM_OK;
 }
 
@@ -233,9 +341,136 @@ int sfmm_launch(sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, const 
         return fail(ctx, SFMM_ERR_UNSUPPORTED, "solidfmm: no kernel for orders");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    e = sfmm::launch_translate(op, p_in, p_out, dev_items, n, src_pool, dst_pool, static_cast<hipStream_t>(stream),
-                               ctx->family);
+    const uint32_t pb = std::max(p_in, p_out);
+    const sfmm::KernelChoice k{family_for(ctx, op, pb), pb, p_in != p_out};
+    e = sfmm::launch_kernel(op, k, dev_items, n, src_pool, dst_pool, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+    return SFMM_OK;
+}
+
+// Autotuner: for each bucket pb of the list, time every family that covers
+// pb on a sample of the bucket's own items (up to kSample, sorted as a plan
+// sorts them) in scratch pools, and keep the fastest for plans created
+// afterwards.
+int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, void *stream)
+{
+    if (!ctx) return SFMM_ERR_LOGIC;
+    if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
+    if (n && !items) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null item list");
+    constexpr size_t kSample = 32768;
+    constexpr int kReps = 5;
+    std::vector<std::vector<sfmm_item>> bucket(kOrders);
+    for (size_t i = 0; i < n; ++i) {
+        const sfmm_item &it = items[i];
+        if (it.p_in == 0 || it.p_out == 0) continue;
+        if (it.p_in > ctx->P || it.p_out > ctx->P)
+            return fail(ctx, SFMM_ERR_RANGE, "solidfmm: orders exceeding operator data.");
+        const uint32_t pb = std::max(it.p_in, it.p_out);
+        if (bucket[pb].size() < kSample) bucket[pb].push_back(it);
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    if (!tables_ready()) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: device tables");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    size_t most = 0;
+    uint32_t pmax = 1;
+    for (uint32_t pb = 1; pb < uint32_t(kOrders); ++pb)
+        if (!bucket[pb].empty()) {
+            most = std::max(most, bucket[pb].size());
+            pmax = pb;
+        }
+    if (most == 0) return SFMM_OK;
+    const size_t block = size_t(pmax) * (pmax + 1);
+    double *src = nullptr, *dst = nullptr;
+    sfmm_item *d_items = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = SFMM_OK;
+    e = hipMalloc(&src, most * block * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dst, most * block * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&d_items, most * sizeof(sfmm_item));
+    if (e != hipSuccess) {
+        rc = fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    } else {
+        // 0x3F bytes: every double is ~4.8e-4 (normal numbers)
+        e = hipMemsetAsync(src, 0x3F, most * block * sizeof(double), s);
+        if (e == hipSuccess) e = hipMemsetAsync(dst, 0x3F, most * block * sizeof(double), s);
+        if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+        if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "autotune setup");
+    }
+    for (uint32_t pb = 1; rc == SFMM_OK && pb < uint32_t(kOrders); ++pb) {
+        std::vector<sfmm_item> &b = bucket[pb];
+        if (b.empty()) continue;
+        std::stable_sort(b.begin(), b.end(), [](const sfmm_item &x, const sfmm_item &y) {
+            return x.p_in != y.p_in ? x.p_in < y.p_in : x.p_out < y.p_out;
+        });
+        bool rt = false;
+        for (size_t i = 0; i < b.size(); ++i) {
+            b[i].src = i * block;
+            b[i].dst = i * block;
+            rt = rt || b[i].p_in != pb || b[i].p_out != pb;
+        }
+        e = hipMemcpyAsync(d_items, b.data(), b.size() * sizeof(sfmm_item), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            rc = hip_fail(ctx, e, "autotune upload");
+            break;
+        }
+        int best = 0;
+        float best_ms = 0.0f;
+        for (int fam : {int(sfmm::kLaneKernels), int(sfmm::kMfmaKernels), int(sfmm::kGenericKernel)}) {
+            if (!sfmm::family_covers(fam, pb)) continue;
+            const sfmm::KernelChoice k{fam, pb, rt};
+            e = sfmm::launch_kernel(op, k, d_items, b.size(), src, dst, s);  // warm-up
+            float ms = 1e30f;
+            for (int r = 0; r < kReps && e == hipSuccess; ++r) {
+                e = hipEventRecord(ev[0], s);
+                if (e == hipSuccess) e = sfmm::launch_kernel(op, k, d_items, b.size(), src, dst, s);
+                if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+                if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+                float t = 0.0f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+                ms = std::min(ms, t);
+            }
+            if (e != hipSuccess) {
+                rc = hip_fail(ctx, e, "autotune launch");
+                break;
+            }
+            if (!best || ms < best_ms) {
+                best = fam;
+                best_ms = ms;
+            }
+        }
+        if (rc == SFMM_OK) ctx->tune[op][pb] = best;
+    }
+    if (ev[0]) (void)hipEventDestroy(ev[0]);
+    if (ev[1]) (void)hipEventDestroy(ev[1]);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (d_items) (void)hipFree(d_items);
+    return rc;
+}
+
+int sfmm_get_tuning(sfmm_context *ctx, int op, int *family_by_order, size_t len)
+{
+    if (!ctx || !family_by_order) return SFMM_ERR_LOGIC;
+    if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (size_t pb = 0; pb < len; ++pb)
+        family_by_order[pb] = (pb == 0 || pb >= size_t(kOrders)) ? 0 : family_for(ctx, op, uint32_t(pb));
+    return SFMM_OK;
+}
+
+int sfmm_set_tuning(sfmm_context *ctx, int op, uint32_t order, int family)
+{
+    if (!ctx) return SFMM_ERR_LOGIC;
+    if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
+    if (order < 1 || order >= uint32_t(kOrders)) return fail(ctx, SFMM_ERR_RANGE, "solidfmm: order out of range");
+    if (family != SFMM_KERNELS_AUTO && !sfmm::family_covers(family, order))
+        return fail(ctx, SFMM_ERR_UNSUPPORTED, "solidfmm: family has no kernel for this order");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tune[op][order] = family;
     return SFMM_OK;
 }
 

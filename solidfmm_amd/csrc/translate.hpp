@@ -1,4 +1,4 @@
-// Internal launcher interface of the translation kernels (translate.hip).
+// Internal launcher interface of the translation kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,37 +10,65 @@
 
 namespace sfmm {
 
-// True when this build has a kernel for (op, p_in, p_out) in `family`.
-bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family = 0);
+// Kernel families:
+//   lane     one lane per translation, FP64 VALU, orders up to 12
+//            (lane_kernel.hpp);
+//   MFMA     one wave per 16 translations, swaps on the FP64 matrix cores,
+//            orders up to 24 (translate_mfma.hpp);
+//   generic  one workgroup per translation, any orders up to 86
+//            (generic.hip).
+// Lane and MFMA kernels are instantiated per square order PB; their RT
+// variants take items of any orders p_in, p_out <= PB.
+enum Family { kAuto = 0, kLaneKernels = 1, kMfmaKernels = 2, kGenericKernel = 3 };
+
+inline constexpr uint32_t kLaneMax = 12;
+inline constexpr uint32_t kLaneAutoMax = 10;  // lane kernels beyond this spill
+
+namespace mfma {
+inline constexpr int kMfmaMin = 1;
+inline constexpr int kMfmaMax = 24;
+template <int OP, bool RT>
+hipError_t launch_op(uint32_t P, const sfmm_item *items, size_t n, const double *src, double *dst,
+                     const double *table, hipStream_t stream);
+}  // namespace mfma
 
 // Highest order with a kernel.
 size_t max_kernel_order();
+
+// True when `family` (not kAuto) has a kernel for items whose larger order
+// is pb.
+bool family_covers(int family, uint32_t pb);
+
+// The family kAuto picks for items whose larger order is pb.
+int default_family(uint32_t pb);
+
+// True when this build has a kernel for (op, p_in, p_out) in `family`.
+bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family = kAuto);
 
 // Workgroups for a persistent kernel: as many as can be resident on the
 // current device at once, but no more than `work` (one per unit of work).
 uint32_t persistent_grid(const void *kernel, int block, size_t work);
 
-// Kernel families: one lane per translation (FP64 VALU, orders up to 12)
-// and one wave per 16 translations (swaps on the FP64 matrix cores).
-// and the generic kernel (one workgroup per translation, any orders).
-enum Family { kAuto = 0, kLaneKernels = 1, kMfmaKernels = 2, kGenericKernel = 3 };
+// One launch of a plan: family (not kAuto), the kernel's order bound pk,
+// and whether items carry their own orders (rt).  The generic kernel
+// always reads per-item orders; pk is then the largest order of the group.
+struct KernelChoice {
+    int family;
+    uint32_t pk;
+    bool rt;
+};
 
-namespace mfma {
-inline constexpr int kMfmaMin = 1;
-inline constexpr int kMfmaMax = 24;
-hipError_t launch_m2m(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
-hipError_t launch_m2l(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
-hipError_t launch_l2l(uint32_t P, const sfmm_item *, size_t, const double *, double *, const double *, hipStream_t);
-}  // namespace mfma
+// Enqueue dst += T(src) for n device-resident items with pairwise distinct
+// targets and orders within `k`.
+hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, size_t n, const double *src,
+                         double *dst, hipStream_t stream);
 
 // The generic kernel: items of any orders up to `pc`.
 hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
                           hipStream_t stream);
 
-// Enqueue dst += T(src) for n device-resident items, all of orders
-// (p_in, p_out) and with pairwise distinct targets.
-// `family` picks the kernel family (kAuto: the lane kernels where they
-// exist, else the MFMA kernels).
+// Enqueue dst += T(src) for n items all of orders (p_in, p_out) with
+// pairwise distinct targets, in `family` (kAuto: default_family).
 hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_item *items, size_t n,
                             const double *src, double *dst, hipStream_t stream, int family = kAuto);
 

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 #include "../../include/solidfmm_gpu.h"
 #include "mfma_tables.hpp"
@@ -178,8 +179,25 @@ __device__ __forceinline__ void transform_bwd(const double *tab, int lane, const
     rot<N, false, false>(R, scale, xr, xi);
 }
 
+// Largest value of v over the wave (0 <= v < 128), wave-uniform: one
+// ballot per bit.
+__device__ __forceinline__ uint32_t wave_max7(uint32_t v)
+{
+    uint32_t m = 0;
+    for (int b = 6; b >= 0; --b) {
+        const uint32_t cand = m | (1u << b);
+        if (__ballot(v >= cand)) m = cand;
+    }
+    return m;
+}
+
 // Translations of one group of 16 (items [16g, 16g+16) of the launch).
-template <int OP, int PI, int PO>
+// RT = false: every item has orders (PI, PO).  RT = true: PI = PO is a
+// bound and each item brings its own p_in, p_out (mixed-order lists); a
+// missing source degree is a zero, output degrees >= p_out (L2L: >=
+// min(p_in, p_out), translations.cpp:412) are not stored, and degrees above
+// the group's largest orders are skipped with wave-uniform branches.
+template <int OP, int PI, int PO, bool RT>
 __device__ __forceinline__ void translate_group(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                                                 const sfmm_item *__restrict__ items, uint32_t n_items, uint32_t g,
                                                 const double *tab, int lane)
@@ -212,6 +230,13 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
         z = 1.0;
     }
     const bool active = valid && !zero_shift;
+    const uint32_t pin = RT ? it.p_in : uint32_t(PI);
+    const uint32_t pout = RT ? it.p_out : uint32_t(PO);
+    const uint32_t pend = pin < pout ? pin : pout;
+    const uint32_t pin_w = RT ? wave_max7(active ? pin : 0u) : uint32_t(PI);
+    const uint32_t pout_w = RT ? wave_max7(active ? (OP == SFMM_L2L ? pend : pout) : 0u) : uint32_t(PO);
+    // does this lane store output degree N?
+    auto out_on = [&](int N) { return !RT || uint32_t(N) < (OP == SFMM_L2L ? pend : pout); };
 
     // Euler data (microkernel_generic.cpp:61-74).
     const double rho2 = x * x + y * y;
@@ -269,7 +294,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             xr[s] = 0.0;
             xi[s] = 0.0;
             if constexpr (4 * s <= n) {
-                if (active && m <= n) {
+                if (active && m <= n && (!RT || uint32_t(n) < pin)) {
                     xr[s] = S[n * (n + 1) + m];
                     xi[s] = S[(n + 1) * (n + 1) + m];
                 }
@@ -286,15 +311,26 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             }
         });
     };
+    auto clear_degree = [&](auto nc) {
+        constexpr int n = decltype(nc)::value;
+        static_for<0, nslots<PI>()>([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            if constexpr (4 * s <= n) {
+                tr[slot_off<PI>(s) + n - 4 * s] = 0.0;
+                ti[slot_off<PI>(s) + n - 4 * s] = 0.0;
+            }
+        });
+    };
     auto fetch_old = [&](auto Nc, double(&orr)[NSL], double(&oi)[NSL]) {
         constexpr int N = decltype(Nc)::value;
+        const bool on = out_on(N);
         static_for<0, NSL>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
             orr[s] = 0.0;
             oi[s] = 0.0;
             if constexpr (4 * s <= N) {
                 const int m = q + 4 * s;
-                if (active && m <= N) {
+                if (active && m <= N && on) {
                     orr[s] = D[N * (N + 1) + m];
                     oi[s] = D[(N + 1) * (N + 1) + m];
                 }
@@ -304,11 +340,12 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
     auto store_sum = [&](auto Nc, const double(&orr)[NSL], const double(&oi)[NSL], const double(&xr)[NSL],
                          const double(&xi)[NSL]) {
         constexpr int N = decltype(Nc)::value;
+        const bool on = out_on(N);
         static_for<0, NSL>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
             if constexpr (4 * s <= N) {
                 const int m = q + 4 * s;
-                if (active && m <= N) {
+                if (active && m <= N && on) {
                     D[N * (N + 1) + m] = orr[s] + xr[s];
                     D[(N + 1) * (N + 1) + m] = oi[s] + xi[s];
                 }
@@ -334,8 +371,12 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             if constexpr (n + 1 < PI) load_degree(ic<n + 1>{}, pr, pim);
             else fetch_old(ic<PO - 1>{}, orr, oi);
             sched_fence();
-            transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
-            keep_degree(nc, xr, xi);
+            if (!RT || uint32_t(n) < pin_w) {
+                transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
+                keep_degree(nc, xr, xi);
+            } else {
+                clear_degree(nc);
+            }
             sc *= rinv;
         });
         double rp[PO];
@@ -351,32 +392,34 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             take(ci, oi);
             if constexpr (N > 0) fetch_old(ic<N - 1>{}, orr, oi);
             sched_fence();
-            double xr[NSL], xi[NSL];
-            const bool odd = ((N + q) & 1) != 0;  // (-1)^(N+m), m = q + 4s
-            static_for<0, NSL>([&](auto sc) {
-                constexpr int s = decltype(sc)::value;
-                double ar = 0.0, ai = 0.0;
-                if constexpr (4 * s <= N && 4 * s < PI) {
-                    static_for<4 * s, PI>([&](auto Kc) {
-                        constexpr int K = decltype(Kc)::value;
-                        constexpr double f = kFac.fac[N + K];
-                        ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
-                        ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
-                    });
-                    if (odd) {
-                        ar = -ar;
-                        ai = -ai;
+            if (!RT || uint32_t(N) < pout_w) {
+                double xr[NSL], xi[NSL];
+                const bool odd = ((N + q) & 1) != 0;  // (-1)^(N+m), m = q + 4s
+                static_for<0, NSL>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    double ar = 0.0, ai = 0.0;
+                    if constexpr (4 * s <= N && 4 * s < PI) {
+                        static_for<4 * s, PI>([&](auto Kc) {
+                            constexpr int K = decltype(Kc)::value;
+                            constexpr double f = kFac.fac[N + K];
+                            ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
+                            ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
+                        });
+                        if (odd) {
+                            ar = -ar;
+                            ai = -ai;
+                        }
+                        if (q + 4 * s > N) {
+                            ar = 0.0;
+                            ai = 0.0;
+                        }
                     }
-                    if (q + 4 * s > N) {
-                        ar = 0.0;
-                        ai = 0.0;
-                    }
-                }
-                xr[s] = ar;
-                xi[s] = ai;
-            });
-            transform_bwd<N, true>(tab, lane, R, rp[N], xr, xi);
-            store_sum(Nc, cr, ci, xr, xi);
+                    xr[s] = ar;
+                    xi[s] = ai;
+                });
+                transform_bwd<N, true>(tab, lane, R, rp[N], xr, xi);
+                store_sum(Nc, cr, ci, xr, xi);
+            }
         });
     } else if constexpr (OP == SFMM_M2M) {
         double sc = rinv;  // forward r^-(n+1)
@@ -398,33 +441,39 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             }
             sched_fence();
             if constexpr (n < PI) {
-                transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
-                keep_degree(nc, xr, xi);
+                if (!RT || uint32_t(n) < pin_w) {
+                    transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
+                    keep_degree(nc, xr, xi);
+                } else {
+                    clear_degree(nc);
+                }
                 sc *= rinv;
             }
             if constexpr (n < PO) {
                 constexpr int N = n;
-                static_for<0, NSL>([&](auto scc) {
-                    constexpr int s = decltype(scc)::value;
-                    double ar = 0.0, ai = 0.0;
-                    if constexpr (4 * s <= N && 4 * s < PI) {
-                        constexpr int KE = (N + 1 < PI) ? N + 1 : PI;
-                        static_for<4 * s, KE>([&](auto Kc) {
-                            constexpr int K = decltype(Kc)::value;
-                            constexpr double f = kFac.inv[N - K];
-                            ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
-                            ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
-                        });
-                        if (q + 4 * s > N) {
-                            ar = 0.0;
-                            ai = 0.0;
+                if (!RT || uint32_t(N) < pout_w) {
+                    static_for<0, NSL>([&](auto scc) {
+                        constexpr int s = decltype(scc)::value;
+                        double ar = 0.0, ai = 0.0;
+                        if constexpr (4 * s <= N && 4 * s < PI) {
+                            constexpr int KE = (N + 1 < PI) ? N + 1 : PI;
+                            static_for<4 * s, KE>([&](auto Kc) {
+                                constexpr int K = decltype(Kc)::value;
+                                constexpr double f = kFac.inv[N - K];
+                                ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
+                                ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
+                            });
+                            if (q + 4 * s > N) {
+                                ar = 0.0;
+                                ai = 0.0;
+                            }
                         }
-                    }
-                    xr[s] = ar;
-                    xi[s] = ai;
-                });
-                transform_bwd<N, false>(tab, lane, R, sb, xr, xi);
-                store_sum(ic<N>{}, cr, ci, xr, xi);
+                        xr[s] = ar;
+                        xi[s] = ai;
+                    });
+                    transform_bwd<N, false>(tab, lane, R, sb, xr, xi);
+                    store_sum(ic<N>{}, cr, ci, xr, xi);
+                }
                 sb *= r;
             }
         });
@@ -459,30 +508,36 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                 fetch_old(ic<PE - 1>{}, orr, oi);
             }
             sched_fence();
-            transform_fwd<n, true>(tab, lane, R, fs[n], xr, xi);
-            keep_degree(nc, xr, xi);
+            if (!RT || uint32_t(n) < pin_w) {
+                transform_fwd<n, true>(tab, lane, R, fs[n], xr, xi);
+                keep_degree(nc, xr, xi);
+            } else {
+                clear_degree(nc);
+            }
             if constexpr (n < PE) {
                 constexpr int N = n;
-                static_for<0, NSL>([&](auto scc) {
-                    constexpr int s = decltype(scc)::value;
-                    double ar = 0.0, ai = 0.0;
-                    if constexpr (4 * s <= N && 4 * s < PI) {
-                        static_for<N, PI>([&](auto Kc) {
-                            constexpr int K = decltype(Kc)::value;
-                            constexpr double f = kFac.inv[K - N];
-                            ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
-                            ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
-                        });
-                        if (q + 4 * s > N) {
-                            ar = 0.0;
-                            ai = 0.0;
+                if (!RT || uint32_t(N) < pout_w) {
+                    static_for<0, NSL>([&](auto scc) {
+                        constexpr int s = decltype(scc)::value;
+                        double ar = 0.0, ai = 0.0;
+                        if constexpr (4 * s <= N && 4 * s < PI) {
+                            static_for<N, PI>([&](auto Kc) {
+                                constexpr int K = decltype(Kc)::value;
+                                constexpr double f = kFac.inv[K - N];
+                                ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
+                                ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
+                            });
+                            if (q + 4 * s > N) {
+                                ar = 0.0;
+                                ai = 0.0;
+                            }
                         }
-                    }
-                    xr[s] = ar;
-                    xi[s] = ai;
-                });
-                transform_bwd<N, true>(tab, lane, R, bs[N], xr, xi);
-                store_sum(ic<N>{}, cr, ci, xr, xi);
+                        xr[s] = ar;
+                        xi[s] = ai;
+                    });
+                    transform_bwd<N, true>(tab, lane, R, bs[N], xr, xi);
+                    store_sum(ic<N>{}, cr, ci, xr, xi);
+                }
             }
         });
     }
@@ -491,14 +546,14 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
         // M2M / L2L with zero shift: target += source over the common
         // degrees (translations.cpp:539-547, 596-604); quarter-wave q takes
         // every fourth real.
-        constexpr int len = PE * (PE + 1);
+        const int len = RT ? int(pend * (pend + 1)) : PE * (PE + 1);
         for (int i = q; i < len; i += 4) D[i] += S[i];
     }
 }
 
 // Persistent workgroups of kWaves waves: stage the swap blocks of degrees
 // < max(PI, PO) in LDS, then each wave takes groups of 16 translations.
-template <int OP, int PI, int PO>
+template <int OP, int PI, int PO, bool RT>
 __global__ void __launch_bounds__(64 * kWaves)
     k_translate_mfma(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                      const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ table)
@@ -513,7 +568,36 @@ __global__ void __launch_bounds__(64 * kWaves)
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t groups = (n_items + 15u) / 16u;
     for (uint32_t g = blockIdx.x * kWaves + wave; g < groups; g += gridDim.x * kWaves)
-        translate_group<OP, PI, PO>(src_pool, dst_pool, items, n_items, g, tab, lane);
+        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab, lane);
+}
+
+using fn_t = void (*)(const double *, double *, const sfmm_item *, uint32_t, const double *);
+
+template <int OP, bool RT, int... Ps>
+constexpr auto make_table(std::integer_sequence<int, Ps...>)
+{
+    struct T {
+        fn_t f[sizeof...(Ps)];
+    } t{{&k_translate_mfma<OP, Ps + kMfmaMin, Ps + kMfmaMin, RT>...}};
+    return t;
+}
+
+// Launch the square-order-P kernel of (OP, RT) over n items as persistent
+// workgroups.  One translation unit instantiates each (OP, RT) pair
+// (mfma_{m2m,m2l,l2l}{,_rt}.hip) so they compile in parallel.
+template <int OP, bool RT>
+hipError_t launch_op(uint32_t P, const sfmm_item *items, size_t n, const double *src, double *dst,
+                     const double *table, hipStream_t stream)
+{
+    static const auto tab = make_table<OP, RT>(std::make_integer_sequence<int, kMfmaMax - kMfmaMin + 1>{});
+    if (P < uint32_t(kMfmaMin) || P > uint32_t(kMfmaMax)) return hipErrorInvalidValue;
+    const fn_t f = tab.f[P - kMfmaMin];
+    const size_t groups = (n + 15) / 16;
+    const uint32_t grid =
+        persistent_grid(reinterpret_cast<const void *>(f), 64 * kWaves, (groups + kWaves - 1) / kWaves);
+    hipLaunchKernelGGL(f, dim3(grid), dim3(64 * kWaves), 0, stream, src, dst, items, static_cast<uint32_t>(n),
+                       table);
+    return hipGetLastError();
 }
 
 }  // namespace mfma

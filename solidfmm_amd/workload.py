@@ -127,20 +127,22 @@ def uniform_batch(op, P_in, P_out, count, seed, shift="m2l", distinct_targets=Tr
     return items, src, dst
 
 
-def mixed_batch(count, seed, pmin=4, pmax=20):
-    """M2L batch with P_in, P_out i.i.d. uniform on {pmin..pmax}, one target each."""
-    pin = pmin + np.minimum((uniform01(seed, 30, count) * (pmax - pmin + 1)).astype(np.int64), pmax - pmin)
-    pout = pmin + np.minimum((uniform01(seed, 31, count) * (pmax - pmin + 1)).astype(np.int64), pmax - pmin)
+def mixed_batch(count, seed, pmin=4, pmax=20, op=M2L):
+    """Batch with P_in, P_out i.i.d. uniform on {pmin..pmax}, one target each,
+    blocks packed back to back.  Shifts as in uniform_batch: |x| in [2,4]
+    for M2L, octant offsets for M2M and L2L."""
+    span = pmax - pmin + 1
+    pin = pmin + np.minimum((uniform01(seed, 30, count) * span).astype(np.int64), span - 1)
+    pout = pmin + np.minimum((uniform01(seed, 31, count) * span).astype(np.int64), span - 1)
     sin = pin * (pin + 1)
     sout = pout * (pout + 1)
     src_off = np.concatenate([[0], np.cumsum(sin)[:-1]]).astype(np.uint64)
     dst_off = np.concatenate([[0], np.cumsum(sout)[:-1]]).astype(np.uint64)
     src = uniform(seed, 32, int(sin.sum()), -1.0, 1.0)
-    for i in range(count):
-        o = int(src_off[i])
-        for n in range(int(pin[i])):
-            src[o + (n + 1) * (n + 1)] = 0.0
+    for n in range(pmax):  # Im C_n^0 = 0 in every block that has degree n
+        has = pin > n
+        src[src_off[has].astype(np.int64) + (n + 1) * (n + 1)] = 0.0
     dst = np.zeros(int(sout.sum()))
-    sh = random_shifts(seed, 40, count, 2.0, 4.0)
+    sh = random_shifts(seed, 40, count, 2.0, 4.0) if op == M2L else octant_shifts(seed, 41, count)
     items = make_items(src_off, dst_off, pin.astype(np.uint32), pout.astype(np.uint32), sh)
     return items, src, dst

@@ -68,3 +68,16 @@ def load_golden(name="golden_f64.npz"):
 
 def orders(items):
     return int(items["p_in"].max()), int(items["p_out"].max())
+
+
+def assert_mixed_parity(got, want, items, scale=1.0, label=""):
+    """assert_parity for mixed orders: each item against tol_fp64 of its own
+    orders (times `scale`); p99 of the ratios <= 1 and max <= 10."""
+    import oracle
+    e = oracle.mixed_errors(got, want, items)
+    tol = np.array([tol_fp64(int(a), int(b)) for a, b in zip(items["p_in"], items["p_out"])]) * scale
+    ratio = e / tol
+    p99 = float(np.percentile(ratio, 99)) if len(ratio) else 0.0
+    mx = float(ratio.max()) if len(ratio) else 0.0
+    assert p99 <= 1.0 and mx <= 10.0, f"{label}: error/tol p99 {p99:.2f} max {mx:.2f}"
+    return p99, mx
