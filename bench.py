@@ -210,6 +210,75 @@ def mixed_mode(args):
     print(json.dumps(line), flush=True)
 
 
+def precision_mode(args):
+    """Precision study: M2L at P in --orders (default 10, 16, 24), --n
+    translations each (default 200,000), in FP64, FP32 and split FP32
+    (double-word).  For each: translations/s over K timed steps and the
+    per-translation error (max over degrees of the per-degree normwise
+    relative error) against the FP64 oracle on a 2,000-item sample."""
+    import torch
+
+    import oracle
+    import solidfmm_amd
+    from solidfmm_amd import workload as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    solidfmm_amd.load()
+    n = args.n if args.n != 100_000 else 200_000
+    orders = [int(p) for p in args.orders.split(",")]
+    stream = torch.cuda.current_stream()
+    orc = oracle.Oracle()
+    results = []
+    for P in orders:
+        items, src, dst = W.uniform_batch(W.M2L, P, P, n, 0x5011DF03 + P)
+        s = torch.from_numpy(src).cuda()
+        idx = np.linspace(0, n - 1, 2000).astype(np.int64)
+        sub = items[idx].copy()
+        sub["dst"] = np.arange(len(idx), dtype=np.uint64) * np.uint64(P * (P + 1))
+        want = orc.translate(W.M2L, sub, src, np.zeros(len(idx) * P * (P + 1)))
+        naive = orc.translate(W.M2L, sub, src, np.zeros(len(idx) * P * (P + 1)), naive=True)
+        spread = oracle.degree_errors(naive, want, P).max(axis=1)
+        for name, precision in solidfmm_amd.PRECISIONS.items():
+            ctx = solidfmm_amd.Context(P, precision=precision)
+            plan = solidfmm_amd.Plan(ctx, W.M2L, items)
+            d = torch.zeros(n * P * (P + 1), dtype=torch.float64, device="cuda")
+            plan.execute(s, d, stream)
+            torch.cuda.synchronize()
+            got = d.cpu().numpy().reshape(n, -1)[idx]
+            e = oracle.degree_errors(got, want.reshape(len(idx), -1), P).max(axis=1)
+            for _ in range(args.warmup):
+                plan.execute(s, d, stream)
+            K = args.steps
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(K):
+                plan.execute(s, d, stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / K
+            fam = plan.launch_info()[0][0]
+            results.append({"P": P, "precision": name,
+                            "kernel": {1: "lane", 2: "mfma", 3: "generic"}[fam],
+                            "translations_per_s": n / (ms * 1e-3), "ms_per_step": ms,
+                            "hbm_frac": n / (ms * 1e-3) * algorithmic_bytes(P, P) / 8e12,
+                            "rel_err_vs_fp64_oracle": {"p50": float(np.percentile(e, 50)),
+                                                       "p99": float(np.percentile(e, 99)),
+                                                       "max": float(e.max())},
+                            "oracle_vs_naive_p99": float(np.percentile(spread, 99))})
+            print(json.dumps(results[-1]), file=sys.stderr, flush=True)
+            del plan, ctx, d
+    head = [r for r in results if r["P"] == orders[0] and r["precision"] == "split32"]
+    line = {"metric": "M2L precision study: translations/s and error vs the FP64 oracle, FP64 / FP32 / split FP32",
+            "value": head[0]["translations_per_s"] if head else None, "unit": "translations/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64/f32/f32x2", "data": "synthetic",
+            "config": {"workload": f"M2L, P in {orders}, {n} translations each, |shift| in [2,4], seeded; FP64 "
+                                   "pools, geometry and accumulation in every mode", "batch": n, "orders": orders},
+            "results": results}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -222,10 +291,11 @@ def main():
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["translate", "mixed", "copy"], default="translate",
+    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "copy"], default="translate",
                     help="mixed: the mixed-order M2L workload (orders --pmin..--pmax, default 500,000 items) "
                          "as one captured HIP graph; copy: time a plain device copy of --copy-mib MiB (HBM "
                          "reference and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--orders", default="10,16,24", help="orders of the precision study")
     ap.add_argument("--pmin", type=int, default=4)
     ap.add_argument("--pmax", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
@@ -241,6 +311,8 @@ def main():
         return copy_mode(args)
     if args.mode == "mixed":
         return mixed_mode(args)
+    if args.mode == "precision":
+        return precision_mode(args)
 
     import torch
 

@@ -43,9 +43,16 @@ extern "C" {
 /* Operators. */
 enum { SFMM_M2M = 0, SFMM_M2L = 1, SFMM_L2L = 2 };
 
-/* Arithmetic of the translation kernels. */
+/* Arithmetic of the translation kernels (sfmm_create).  Pools are FP64 in
+ * every mode; geometry (Euler angles, their powers, the r^n scale tables)
+ * and the accumulation into the target are FP64.  The reduced modes run the
+ * bulk arithmetic (rotations, swaps, z-translation) in FP32 or in
+ * double-word FP32 (hi/lo float pairs, about 48 bits) and support orders up
+ * to 24 (SFMM_ERR_OVERFLOW above). */
 enum {
-    SFMM_FP64 = 0 /* FP64 throughout (the reference's double path) */
+    SFMM_FP64 = 0,   /* FP64 throughout (the reference's double path) */
+    SFMM_FP32 = 1,   /* FP32 bulk arithmetic */
+    SFMM_SPLIT32 = 2 /* double-word FP32 bulk arithmetic */
 };
 
 /* Kernel families (sfmm_set_kernels, sfmm_set_tuning).  Lane and MFMA
@@ -86,9 +93,14 @@ typedef struct sfmm_plan sfmm_plan;
 /* Highest order this build supports (FP64): 86, the reference's limit. */
 size_t sfmm_max_order(void);
 
-/* Context on HIP device `device` for orders up to P.  Replaces
+/* Context on HIP device `device` for orders up to P, with the arithmetic
+ * `precision` (SFMM_FP64, SFMM_FP32 or SFMM_SPLIT32).  Replaces
  * operator_handle<real>(P) (handles.hpp:40, handles.cpp:42-45): P == 0 is
- * SFMM_ERR_LOGIC, P above sfmm_max_order() is SFMM_ERR_OVERFLOW. */
+ * SFMM_ERR_LOGIC, P above sfmm_max_order() (24 for the reduced modes) is
+ * SFMM_ERR_OVERFLOW.  Reduced-precision contexts run square M2L of orders
+ * <= 12 (FP32) or <= 10 (split) in a one-lane-per-translation kernel and
+ * everything else in a one-workgroup-per-translation kernel (no MFMA
+ * family, no autotuning). */
 int sfmm_create(sfmm_context **ctx, int device, size_t P, int precision);
 void sfmm_destroy(sfmm_context *ctx);
 size_t sfmm_order(const sfmm_context *ctx);

@@ -9,6 +9,7 @@ mirrors the reference's multithreaded_translator batch interface
 """
 from . import workload
 from .workload import ITEM_DTYPE, M2M, M2L, L2L, solid_size
+from ._native import (SFMM_FP64, SFMM_FP32, SFMM_SPLIT32, PRECISIONS)
 from ._native import (Context, Plan, load, LIB_PATH, SolidfmmError, LogicError, OutOfRangeError,
                       OrderOverflowError, DeviceAllocError, HipError, UnsupportedError)
 
@@ -41,6 +42,6 @@ class Translator:
         return self._run(L2L, items, src, dst, stream)
 
 
-__all__ = ["workload", "ITEM_DTYPE", "M2M", "M2L", "L2L", "solid_size", "Context", "Plan", "Translator",
-           "load", "LIB_PATH", "SolidfmmError", "LogicError", "OutOfRangeError", "OrderOverflowError",
+__all__ = ["SFMM_FP64", "SFMM_FP32", "SFMM_SPLIT32", "PRECISIONS", "workload", "ITEM_DTYPE", "M2M", "M2L", "L2L",
+           "solid_size", "Context", "Plan", "Translator", "load", "LIB_PATH", "SolidfmmError", "LogicError", "OutOfRangeError", "OrderOverflowError",
            "DeviceAllocError", "HipError", "UnsupportedError"]

@@ -22,6 +22,9 @@ SFMM_ERR_HIP = -5
 SFMM_ERR_UNSUPPORTED = -6
 
 SFMM_FP64 = 0
+SFMM_FP32 = 1
+SFMM_SPLIT32 = 2
+PRECISIONS = {"fp64": SFMM_FP64, "fp32": SFMM_FP32, "split32": SFMM_SPLIT32}
 
 KERNELS_AUTO, KERNELS_LANE, KERNELS_MFMA, KERNELS_GENERIC = 0, 1, 2, 3
 
@@ -145,7 +148,11 @@ def _dev_ptr(t, dtype="float64"):
 
 
 class Context:
-    """Device context for orders up to P (operator_handle in the reference)."""
+    """Device context for orders up to P (operator_handle in the reference).
+
+    ``precision``: SFMM_FP64 (default), SFMM_FP32 or SFMM_SPLIT32 (bulk
+    arithmetic in FP32 or double-word FP32; FP64 pools, geometry and
+    accumulation; orders up to 24)."""
 
     def __init__(self, P, device=0, precision=SFMM_FP64):
         lib = load()

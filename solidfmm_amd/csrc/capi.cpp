@@ -81,6 +81,32 @@ int family_for(const sfmm_context *ctx, int op, uint32_t pb)
     return t ? t : sfmm::default_family(pb);
 }
 
+// Kernel for one item on a context: family and whether it has one.  FP64
+// contexts: the family of its bucket max(p_in, p_out).  Reduced-precision
+// contexts: the precision lane kernel for square M2L up to kPrecLaneMax
+// (kPrecLaneMaxDW for double-word FP32) unless the generic family is
+// forced, else the precision generic kernel.
+bool choose(const sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, int &family)
+{
+    const uint32_t pb = std::max(p_in, p_out);
+    if (ctx->precision == SFMM_FP64) {
+        if (!sfmm::have_kernel(op, p_in, p_out, ctx->family)) return false;
+        family = family_for(ctx, op, pb);
+        return true;
+    }
+    if (pb > sfmm::kPrecMaxOrder) return false;
+    const uint32_t lane_max = ctx->precision == SFMM_SPLIT32 ? sfmm::kPrecLaneMaxDW : sfmm::kPrecLaneMax;
+    const bool lane_ok = op == SFMM_M2L && p_in == p_out && pb <= lane_max;
+    switch (ctx->family) {
+    case SFMM_KERNELS_LANE:
+        family = sfmm::kLaneKernels;
+        return lane_ok;
+    case SFMM_KERNELS_MFMA: return false;
+    case SFMM_KERNELS_GENERIC: family = sfmm::kGenericKernel; return true;
+    default: family = lane_ok ? sfmm::kLaneKernels : sfmm::kGenericKernel; return true;
+    }
+}
+
 // Validation and grouping.  Items with an order of zero contribute nothing
 // (the reference pads such lanes with zeros and writes them to its trash
 // buffer, microkernel_generic.cpp:340-380) and are dropped.
@@ -107,12 +133,12 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
         if ((it.src | it.dst) & 1u)
             return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: expansion offsets must be even (16-byte aligned)");
         if (it.p_in == 0 || it.p_out == 0) continue;
-        if (!sfmm::have_kernel(op, it.p_in, it.p_out, ctx->family))
+        int fam = 0;
+        if (!choose(ctx, op, it.p_in, it.p_out, fam))
             return fail(ctx, SFMM_ERR_UNSUPPORTED,
                         std::string("solidfmm: no kernel for ") + op_name(op) + " with p_in=" +
                             std::to_string(it.p_in) + ", p_out=" + std::to_string(it.p_out));
         const uint32_t pb = std::max(it.p_in, it.p_out);
-        const int fam = family_for(ctx, op, pb);
         const uint32_t round = seen[it.dst]++;
         // the generic kernel takes every order in one launch
         keys.push_back({round, fam, fam == sfmm::kGenericKernel ? 0u : pb, it.p_in, it.p_out, i});
@@ -131,7 +157,7 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
         const Key &key = keys[k];
         if (k == 0 || keys[k - 1].round != key.round || keys[k - 1].family != key.family ||
             keys[k - 1].pb != key.pb)
-            launches.push_back({{key.family, key.pb, false}, ordered.size(), 0});
+            launches.push_back({{key.family, key.pb, false, ctx->precision}, ordered.size(), 0});
         sfmm_plan::Launch &L = launches.back();
         if (key.family == sfmm::kGenericKernel) {
             L.k.pk = std::max({L.k.pk, key.p_in, key.p_out});
@@ -174,8 +200,9 @@ int sfmm_create(sfmm_context **out, int device, size_t P, int precision)
     if (!out) return SFMM_ERR_LOGIC;
     *out = nullptr;
     if (P == 0) return SFMM_ERR_LOGIC;
-    if (P > sfmm::max_kernel_order()) return SFMM_ERR_OVERFLOW;
-    if (precision != SFMM_FP64) return SFMM_ERR_UNSUPPORTED;
+    if (precision != SFMM_FP64 && precision != SFMM_FP32 && precision != SFMM_SPLIT32) return SFMM_ERR_LOGIC;
+    if (P > (precision == SFMM_FP64 ? sfmm::max_kernel_order() : size_t(sfmm::kPrecMaxOrder)))
+        return SFMM_ERR_OVERFLOW;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return SFMM_ERR_HIP;
     sfmm_context *ctx = new (std::nothrow) sfmm_context;
@@ -337,12 +364,12 @@ int sfmm_launch(sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, const 
     if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
     if (p_in > ctx->P || p_out > ctx->P) return fail(ctx, SFMM_ERR_RANGE, "solidfmm: orders exceeding operator data.");
     if (n == 0 || p_in == 0 || p_out == 0) return SFMM_OK;
-    if (!sfmm::have_kernel(op, p_in, p_out, ctx->family))
-        return fail(ctx, SFMM_ERR_UNSUPPORTED, "solidfmm: no kernel for orders");
+    int fam = 0;
+    if (!choose(ctx, op, p_in, p_out, fam)) return fail(ctx, SFMM_ERR_UNSUPPORTED, "solidfmm: no kernel for orders");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     const uint32_t pb = std::max(p_in, p_out);
-    const sfmm::KernelChoice k{family_for(ctx, op, pb), pb, p_in != p_out};
+    const sfmm::KernelChoice k{fam, pb, p_in != p_out, ctx->precision};
     e = sfmm::launch_kernel(op, k, dev_items, n, src_pool, dst_pool, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
     return SFMM_OK;
@@ -357,6 +384,7 @@ int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, v
     if (!ctx) return SFMM_ERR_LOGIC;
     if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
     if (n && !items) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null item list");
+    if (ctx->precision != SFMM_FP64) return SFMM_OK;  // one kernel per case: nothing to choose
     constexpr size_t kSample = 32768;
     constexpr int kReps = 5;
     std::vector<std::vector<sfmm_item>> bucket(kOrders);

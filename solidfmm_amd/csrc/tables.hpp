@@ -103,6 +103,29 @@ constexpr FacTable make_fac()
 
 inline constexpr FacTable kFac = make_fac();
 
+// 2^k, exact.
+constexpr double pow2(int k)
+{
+    double v = 1.0;
+    for (; k > 0; --k) v *= 2.0;
+    for (; k < 0; ++k) v *= 0.5;
+    return v;
+}
+
+// floor(log2((n-m)! (n+m)!) / 2): 2^enorm(n, m) approximates the norm
+// sqrt((n-m)! (n+m)!) that makes the swap blocks of degree n near-orthogonal
+// (the reduced-precision kernels work in that scaled basis).
+constexpr int enorm(int n, int m)
+{
+    double p = kFac.fac[n - m] * kFac.fac[n + m];
+    int e = 0;
+    while (p >= 2.0) {
+        p *= 0.5;
+        ++e;
+    }
+    return e >> 1;
+}
+
 template <int I>
 using ic = std::integral_constant<int, I>;
 

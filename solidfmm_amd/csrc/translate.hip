@@ -92,6 +92,12 @@ hipError_t launch_mfma(int op, uint32_t pk, const sfmm_item *items, size_t n, co
 hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, size_t n, const double *src,
                          double *dst, hipStream_t stream)
 {
+    if (k.precision != SFMM_FP64) {
+        if (k.family == kLaneKernels && op == SFMM_M2L && !k.rt)
+            return launch_prec_lane_m2l(k.precision, k.pk, items, n, src, dst, stream);
+        if (k.family == kGenericKernel) return launch_prec_generic(k.precision, op, k.pk, items, n, src, dst, stream);
+        return hipErrorInvalidValue;
+    }
     if (op < 0 || op > 2 || !family_covers(k.family, k.pk)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;

@@ -49,13 +49,26 @@ bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family = kAuto);
 // current device at once, but no more than `work` (one per unit of work).
 uint32_t persistent_grid(const void *kernel, int block, size_t work);
 
+// Reduced-precision kernels (arithmetic policies of prec.hpp): a lane
+// kernel for square M2L up to kPrecLaneMax and a generic kernel for every
+// operator and order up to kPrecMaxOrder.
+inline constexpr uint32_t kPrecLaneMax = 12;    // FP32
+inline constexpr uint32_t kPrecLaneMaxDW = 10;  // double-word FP32
+inline constexpr uint32_t kPrecMaxOrder = 24;
+hipError_t launch_prec_lane_m2l(int precision, uint32_t P, const sfmm_item *items, size_t n, const double *src,
+                                double *dst, hipStream_t stream);
+hipError_t launch_prec_generic(int precision, int op, uint32_t pc, const sfmm_item *items, size_t n,
+                               const double *src, double *dst, hipStream_t stream);
+
 // One launch of a plan: family (not kAuto), the kernel's order bound pk,
-// and whether items carry their own orders (rt).  The generic kernel
-// always reads per-item orders; pk is then the largest order of the group.
+// whether items carry their own orders (rt), and the arithmetic
+// (SFMM_FP64, SFMM_FP32, SFMM_SPLIT32).  The generic kernels always read
+// per-item orders; pk is then the largest order of the group.
 struct KernelChoice {
     int family;
     uint32_t pk;
     bool rt;
+    int precision = SFMM_FP64;
 };
 
 // Enqueue dst += T(src) for n device-resident items with pairwise distinct
