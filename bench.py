@@ -279,6 +279,69 @@ def precision_mode(args):
     print(json.dumps(line), flush=True)
 
 
+def serving_mode(args):
+    """Serving: the FMM passes (M2M up, M2L, L2L down) of an adaptive octree
+    over a Plummer sphere (--points, at most 64 per leaf; orders 20 at the
+    root to 6 at the leaves) cut into requests of U[1k, 50k] translations.
+    One-at-a-time capacity is measured first (each request alone,
+    synchronised); then the requests arrive as a Poisson process at
+    --load x that capacity and are served by the continuous-batching loop
+    (--slots streams, --budget-mib device arena).  Reports translations/s and
+    p50/p99 request latency per precision, plus a saturated run (every
+    request available at once)."""
+    import torch
+
+    import solidfmm_amd
+    from solidfmm_amd import octree as O
+    from solidfmm_amd import serving as S
+    from solidfmm_amd import workload as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    solidfmm_amd.load()
+    seed = 0x5011DF04
+    t0 = time.perf_counter()
+    tree = O.Octree(O.plummer(args.points, seed), leaf_size=64)
+    P, off, pool_len, passes = O.translation_lists(tree, 20, 6)
+    m0 = W.uniform(seed, 7, pool_len, -1.0, 1.0)
+    build_s = time.perf_counter() - t0
+    results = []
+    for name in args.precisions.split(","):
+        precision = solidfmm_amd.PRECISIONS[name]
+        ctx = solidfmm_amd.Context(20, precision=precision)
+        pools = (torch.from_numpy(m0).cuda(), torch.zeros(pool_len, dtype=torch.float64, device="cuda"))
+        reqs = S.make_requests(passes, seed)
+        S.run_sequential(ctx, pools, reqs[:20])  # warm-up
+        gpu_ms = S.run_sequential(ctx, pools, reqs)
+        total = sum(len(r.items) for r in reqs)
+        cap_rps = len(reqs) / (sum(gpu_ms) * 1e-3)
+        row = {"precision": name, "requests": len(reqs), "translations": total,
+               "alone_gpu_ms": sum(gpu_ms), "alone_translations_per_s": total / (sum(gpu_ms) * 1e-3)}
+        server = S.Server(ctx, pools, len(tree), slots=args.slots, budget_bytes=args.budget_mib << 20)
+        reqs = S.make_requests(passes, seed)
+        row["saturated"] = server.run(reqs, realtime=False)
+        reqs = S.make_requests(passes, seed)
+        S.poisson_arrivals(reqs, args.load * cap_rps, seed)
+        row["rate_requests_per_s"] = args.load * cap_rps
+        row["poisson"] = server.run(reqs, realtime=True)
+        results.append(row)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+        del server, pools, ctx
+    head = results[0]["poisson"]
+    line = {"metric": "serving: translations/s and p50/p99 request latency, FMM passes of a Plummer octree",
+            "value": head["translations_per_s"], "unit": "translations/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64 / f32x2",
+            "data": "synthetic",
+            "config": {"workload": f"Plummer sphere of {args.points} points (a=1, seeded), octree <= 64 per leaf "
+                                   f"({len(tree)} cells, depth {tree.depth}), orders 20 (root) to 6 (leaves); "
+                                   "M2M/M2L/L2L passes cut into requests of U[1k,50k] translations, Poisson "
+                                   f"arrivals at {args.load} x one-at-a-time capacity; {args.slots} streams, "
+                                   f"{args.budget_mib} MiB device arena for requests in flight",
+                       "points": args.points, "cells": int(len(tree)), "depth": int(tree.depth),
+                       "tree_and_lists_s": build_s},
+            "latency_ms": head["latency_ms"], "results": results}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,11 +354,17 @@ def main():
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "copy"], default="translate",
+    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "serving", "copy"], default="translate",
                     help="mixed: the mixed-order M2L workload (orders --pmin..--pmax, default 500,000 items) "
                          "as one captured HIP graph; copy: time a plain device copy of --copy-mib MiB (HBM "
                          "reference and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--orders", default="10,16,24", help="orders of the precision study")
+    ap.add_argument("--points", type=int, default=1_000_000, help="serving: Plummer points")
+    ap.add_argument("--load", type=float, default=0.8,
+                    help="serving: Poisson arrival rate as a fraction of the measured one-at-a-time capacity")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-mib", type=int, default=1024)
+    ap.add_argument("--precisions", default="fp64,split32")
     ap.add_argument("--pmin", type=int, default=4)
     ap.add_argument("--pmax", type=int, default=20)
     ap.add_argument("--no-autotune", action="store_true")
@@ -313,6 +382,8 @@ def main():
         return mixed_mode(args)
     if args.mode == "precision":
         return precision_mode(args)
+    if args.mode == "serving":
+        return serving_mode(args)
 
     import torch
 

@@ -113,13 +113,32 @@ int sfmm_set_kernels(sfmm_context *ctx, int family);
 /* Plan: a validated, device-resident translation list.  Replaces the
  * validation and queueing of m2m/m2l/l2l (translations.cpp:634-1001): every
  * item is checked before anything is written (orders <= context order,
- * else SFMM_ERR_RANGE; odd offsets SFMM_ERR_LOGIC).  Items sharing a target
- * are applied in list order ("rounds": the k-th item of a target runs in
- * round k).  Within a round, items are bucketed by max(p_in, p_out) and
- * the family chosen for that bucket (sfmm_set_kernels, else the autotuned
- * family, else the default); each bucket is one launch, its items sorted by
- * (p_in, p_out). */
+ * else SFMM_ERR_RANGE; odd offsets SFMM_ERR_LOGIC).  Items are bucketed by
+ * max(p_in, p_out) and the family chosen for that bucket (sfmm_set_kernels,
+ * else the autotuned family, else the default); each bucket is one launch,
+ * its items sorted by (p_in, p_out).  Items sharing a target: if no target
+ * occurs more than 4 times they are applied in list order by rounds (the
+ * k-th item of a target runs in round k); otherwise (e.g. M2L interaction
+ * lists) every item writes its own scratch block and a segmented reduction
+ * adds each target's blocks, in list order, into the target once.  Results
+ * are bitwise reproducible either way. */
 int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, sfmm_plan **plan);
+/* Two-step plan creation for request loops that carve device memory out of
+ * one arena under an HBM budget (no counterpart in the reference).
+ * sfmm_plan_prepare validates and groups like sfmm_plan_create but touches
+ * no device and holds the context's lock only to read its settings, so
+ * several host threads can prepare plans while kernels run.
+ * sfmm_plan_device_bytes is the device memory the plan needs (its items,
+ * plus reduction tables and scratch when targets repeat).  sfmm_plan_bind
+ * then copies the plan's tables into caller device memory `dev_mem` (at
+ * least sfmm_plan_device_bytes bytes, 256-byte aligned) asynchronously on
+ * `stream`; the plan allocates and frees no device memory, so nothing
+ * waits for the device.  Execute a bound plan on the same stream (or after
+ * the copy), and keep `dev_mem` to the plan until its last launch has
+ * completed. */
+int sfmm_plan_prepare(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, sfmm_plan **plan);
+size_t sfmm_plan_device_bytes(const sfmm_plan *plan);
+int sfmm_plan_bind(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *stream);
 void sfmm_plan_destroy(sfmm_plan *plan);
 size_t sfmm_plan_launches(const sfmm_plan *plan);
 

@@ -48,6 +48,9 @@ SIGNATURES = {
     "sfmm_last_error": (_c.c_char_p, [_vp]),
     "sfmm_set_kernels": (_c.c_int, [_vp, _c.c_int]),
     "sfmm_plan_create": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
+    "sfmm_plan_prepare": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
+    "sfmm_plan_bind": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
+    "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
     "sfmm_plan_destroy": (None, [_vp]),
     "sfmm_plan_launches": (_c.c_size_t, [_vp]),
     "sfmm_plan_extents": (None, [_vp, _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
@@ -226,17 +229,32 @@ class Context:
 
 
 class Plan:
-    """A validated translation list resident on the device."""
+    """A validated translation list resident on the device.
 
-    def __init__(self, ctx, op, items):
+    With ``prepared=True`` the plan is only validated and grouped on the host
+    (sfmm_plan_prepare); ``bind(dev_ptr, nbytes, stream)`` then copies its
+    tables into caller device memory of ``device_bytes`` bytes
+    asynchronously (sfmm_plan_bind)."""
+
+    def __init__(self, ctx, op, items, prepared=False):
         items = np.ascontiguousarray(items, dtype=ITEM_DTYPE)
         self.ctx = ctx
         self.op = op
         self.n = len(items)
         h = ctypes.c_void_p()
-        rc = load().sfmm_plan_create(ctx.handle, op, items.ctypes.data, len(items), ctypes.byref(h))
-        _check(rc, ctx.handle, "sfmm_plan_create")
+        f = load().sfmm_plan_prepare if prepared else load().sfmm_plan_create
+        rc = f(ctx.handle, op, items.ctypes.data, len(items), ctypes.byref(h))
+        _check(rc, ctx.handle, "sfmm_plan_prepare" if prepared else "sfmm_plan_create")
         self._h = h
+
+    @property
+    def device_bytes(self):
+        """Device memory the plan needs (items, and reduction tables plus
+        scratch when targets repeat)."""
+        return load().sfmm_plan_device_bytes(self._h)
+
+    def bind(self, ptr, nbytes, stream=None):
+        _check(load().sfmm_plan_bind(self._h, ptr, nbytes, _stream_ptr(stream)), self.ctx.handle, "sfmm_plan_bind")
 
     @property
     def launches(self):

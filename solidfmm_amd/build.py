@@ -27,7 +27,7 @@ COMMON = [
 # Translation units; the kernel instantiations are split so they compile in
 # parallel.
 SOURCES = ["lane_prec_dw.hip", "mfma_m2l_rt.hip", "mfma_m2l.hip", "mfma_m2m_rt.hip", "mfma_l2l_rt.hip", "mfma_m2m.hip",
-           "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "translate.hip", "generic.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp"]
+           "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "reduce.hip", "translate.hip", "generic.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp"]
 
 # Host-only C++ (the drop-in API over the C ABI).
 HOST_ONLY = {"dropin.cpp"}
@@ -48,10 +48,25 @@ def _compile(src):
     return obj
 
 
-def build(verbose=True):
+def _stale(src):
+    """True when src's object is missing or older than src or any header."""
+    obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps += [os.path.join(INCLUDE, "solidfmm_gpu.h")]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose=True, incremental=False):
+    """Compile every source (incremental: only stale ones) and link."""
     os.makedirs(OBJ_DIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    todo = [s for s in SOURCES if _stale(s)] if incremental else list(SOURCES)
+    if todo:
+        with cf.ThreadPoolExecutor(max_workers=min(os.cpu_count() or 8, len(todo))) as ex:
+            list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJ_DIR, os.path.splitext(s)[0] + ".o") for s in SOURCES]
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -64,5 +79,5 @@ def build(verbose=True):
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -51,12 +52,21 @@ struct sfmm_plan {
     };
     sfmm_context *ctx = nullptr;
     int op = 0;
-    sfmm_item *d_items = nullptr;
     size_t n_items = 0;
     size_t src_extent = 0, dst_extent = 0;
     std::vector<Launch> launches;
+    // Device memory, one block: items | reduction targets | reduction items
+    // | scratch.  The first three are uploaded from `blob`.
+    bool scratch = false;  // targets repeat: per-item scratch + reduction
+    size_t n_targets = 0;
+    size_t off_targets = 0, off_red_items = 0, off_scratch = 0, dev_bytes = 0;
+    std::vector<unsigned char> blob;
+    unsigned char *d_mem = nullptr;
+    bool owns_mem = true;  // false: caller memory (sfmm_plan_bind)
     // captured graph (sfmm_plan_capture)
     hipGraphExec_t exec = nullptr;
+
+    const sfmm_item *d_items() const { return reinterpret_cast<const sfmm_item *>(d_mem); }
 };
 
 namespace {
@@ -107,11 +117,29 @@ bool choose(const sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, int 
     }
 }
 
-// Validation and grouping.  Items with an order of zero contribute nothing
-// (the reference pads such lanes with zeros and writes them to its trash
-// buffer, microkernel_generic.cpp:340-380) and are dropped.
-int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, std::vector<sfmm_item> &ordered,
-                 std::vector<sfmm_plan::Launch> &launches)
+// Targets repeated more often than this are accumulated through scratch
+// blocks and a segmented reduction instead of one launch per repeat.
+constexpr uint32_t kMaxRounds = 4;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Reals a translation writes into its (scratch) target block.
+uint32_t written_reals(int op, const sfmm_item &it)
+{
+    const uint32_t pend = std::min(it.p_in, it.p_out);
+    const bool zero = it.x == 0.0 && it.y == 0.0 && it.z == 0.0;
+    if (op == SFMM_L2L || (op == SFMM_M2M && zero)) return pend * (pend + 1);
+    return it.p_out * (it.p_out + 1);
+}
+
+// Validation and grouping into `plan` (host side only).  Items with an
+// order of zero contribute nothing (the reference pads such lanes with
+// zeros and writes them to its trash buffer, microkernel_generic.cpp:
+// 340-380) and are dropped.  Targets that repeat at most kMaxRounds times
+// are applied in list order by rounds (the k-th item of a target runs in
+// launch round k); otherwise every item writes its own scratch block and a
+// segmented reduction adds a target's blocks in list order (reduce.hip).
+int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm_plan &plan)
 {
     if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
     if (n && !items) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null item list");
@@ -123,8 +151,9 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
     };
     std::vector<Key> keys;
     keys.reserve(n);
-    std::unordered_map<uint64_t, uint32_t> seen;
+    std::unordered_map<uint64_t, uint32_t> seen;  // target -> occurrences so far
     seen.reserve(n * 2 + 1);
+    uint32_t rounds = 0;
     for (size_t i = 0; i < n; ++i) {
         const sfmm_item &it = items[i];
         if (it.p_in > ctx->P || it.p_out > ctx->P)
@@ -140,8 +169,20 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
                             std::to_string(it.p_in) + ", p_out=" + std::to_string(it.p_out));
         const uint32_t pb = std::max(it.p_in, it.p_out);
         const uint32_t round = seen[it.dst]++;
+        rounds = std::max(rounds, round + 1);
+        plan.src_extent = std::max<size_t>(plan.src_extent, it.src + size_t(it.p_in) * (it.p_in + 1));
+        plan.dst_extent = std::max<size_t>(plan.dst_extent, it.dst + size_t(it.p_out) * (it.p_out + 1));
         // the generic kernel takes every order in one launch
         keys.push_back({round, fam, fam == sfmm::kGenericKernel ? 0u : pb, it.p_in, it.p_out, i});
+    }
+    plan.scratch = rounds > kMaxRounds;
+    if (plan.scratch)
+        for (Key &k : keys) k.round = 0;
+    // keys are in list order here: remember each item's position in it
+    std::vector<uint32_t> list_pos;
+    if (plan.scratch) {
+        list_pos.resize(n);
+        for (size_t k = 0; k < keys.size(); ++k) list_pos[keys[k].index] = uint32_t(k);
     }
     std::stable_sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
         if (a.round != b.round) return a.round < b.round;
@@ -150,22 +191,86 @@ int build_groups(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, st
         if (a.p_in != b.p_in) return a.p_in < b.p_in;
         return a.p_out < b.p_out;
     });
-    ordered.clear();
-    ordered.reserve(keys.size());
-    launches.clear();
-    for (size_t k = 0; k < keys.size(); ++k) {
+    const size_t m = keys.size();
+    plan.n_items = m;
+    plan.launches.clear();
+    std::vector<sfmm_item> ordered;
+    ordered.reserve(m);
+    std::vector<uint64_t> scr_off;  // scratch offset per list position
+    size_t scratch_reals = 0;
+    if (plan.scratch) scr_off.resize(m);
+    for (size_t k = 0; k < m; ++k) {
         const Key &key = keys[k];
         if (k == 0 || keys[k - 1].round != key.round || keys[k - 1].family != key.family ||
             keys[k - 1].pb != key.pb)
-            launches.push_back({{key.family, key.pb, false, ctx->precision}, ordered.size(), 0});
-        sfmm_plan::Launch &L = launches.back();
+            plan.launches.push_back({{key.family, key.pb, false, ctx->precision, !plan.scratch}, ordered.size(), 0});
+        sfmm_plan::Launch &L = plan.launches.back();
         if (key.family == sfmm::kGenericKernel) {
             L.k.pk = std::max({L.k.pk, key.p_in, key.p_out});
         } else if (key.p_in != key.pb || key.p_out != key.pb) {
             L.k.rt = true;
         }
-        ordered.push_back(items[key.index]);
+        sfmm_item it = items[key.index];
+        if (plan.scratch) {  // the item's own block in the scratch pool
+            scr_off[list_pos[key.index]] = scratch_reals;
+            it.dst = scratch_reals;
+            scratch_reals += size_t(it.p_out) * (it.p_out + 1);
+        }
+        ordered.push_back(it);
         L.count++;
+    }
+    // Reduction tables: targets in order of first appearance, each with its
+    // items in list order.
+    std::vector<sfmm::RedTarget> rt;
+    std::vector<sfmm::RedItem> ri;
+    if (plan.scratch) {
+        std::unordered_map<uint64_t, uint32_t> tid;
+        tid.reserve(seen.size() * 2 + 1);
+        std::vector<uint32_t> count;
+        std::vector<uint32_t> of_pos(m);  // target id per list position
+        size_t pos = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const sfmm_item &it = items[i];
+            if (it.p_in == 0 || it.p_out == 0) continue;
+            auto ins = tid.emplace(it.dst, uint32_t(rt.size()));
+            if (ins.second) {
+                rt.push_back({it.dst, 0, 0, 0, 0});
+                count.push_back(0);
+            }
+            const uint32_t t = ins.first->second;
+            of_pos[pos++] = t;
+            count[t]++;
+            rt[t].len = std::max(rt[t].len, written_reals(op, it));
+        }
+        uint32_t first = 0;
+        for (size_t t = 0; t < rt.size(); ++t) {
+            rt[t].first = first;
+            first += count[t];
+            count[t] = 0;
+        }
+        ri.resize(m);
+        pos = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const sfmm_item &it = items[i];
+            if (it.p_in == 0 || it.p_out == 0) continue;
+            const uint32_t t = of_pos[pos];
+            ri[rt[t].first + count[t]++] = {scr_off[pos], written_reals(op, it), 0};
+            ++pos;
+        }
+        for (size_t t = 0; t < rt.size(); ++t) rt[t].count = count[t];
+    }
+    plan.n_targets = rt.size();
+    // device layout and the host image of its first three parts
+    plan.off_targets = align_up(m * sizeof(sfmm_item), 256);
+    plan.off_red_items = align_up(plan.off_targets + rt.size() * sizeof(sfmm::RedTarget), 256);
+    plan.off_scratch = align_up(plan.off_red_items + ri.size() * sizeof(sfmm::RedItem), 256);
+    plan.dev_bytes = plan.scratch ? plan.off_scratch + scratch_reals * sizeof(double) : m * sizeof(sfmm_item);
+    const size_t image = plan.scratch ? plan.off_scratch : m * sizeof(sfmm_item);
+    plan.blob.assign(image, 0);
+    if (m) std::memcpy(plan.blob.data(), ordered.data(), m * sizeof(sfmm_item));
+    if (plan.scratch) {
+        std::memcpy(plan.blob.data() + plan.off_targets, rt.data(), rt.size() * sizeof(sfmm::RedTarget));
+        std::memcpy(plan.blob.data() + plan.off_red_items, ri.data(), ri.size() * sizeof(sfmm::RedItem));
     }
     return SFMM_OK;
 }
@@ -178,10 +283,16 @@ void drop_graph(sfmm_plan *plan)
 
 hipError_t enqueue(sfmm_plan *plan, const double *src, double *dst, hipStream_t stream)
 {
+    double *out = plan->scratch ? reinterpret_cast<double *>(plan->d_mem + plan->off_scratch) : dst;
     for (const auto &L : plan->launches) {
-        hipError_t e = sfmm::launch_kernel(plan->op, L.k, plan->d_items + L.offset, L.count, src, dst, stream);
+        hipError_t e = sfmm::launch_kernel(plan->op, L.k, plan->d_items() + L.offset, L.count, src, out, stream);
         if (e != hipSuccess) return e;
     }
+    if (plan->scratch)
+        return sfmm::launch_reduce(reinterpret_cast<const sfmm::RedTarget *>(plan->d_mem + plan->off_targets),
+                                   plan->n_targets,
+                                   reinterpret_cast<const sfmm::RedItem *>(plan->d_mem + plan->off_red_items), out,
+                                   dst, stream);
     return hipSuccess;
 }
 
@@ -230,41 +341,109 @@ int sfmm_set_kernels(sfmm_context *ctx, int family)
     return SFMM_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Validation and grouping into a new plan with no device memory yet (its
+// device image stays on the host).
+int prepare_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm_plan **out)
+{
+    sfmm_plan *plan = new (std::nothrow) sfmm_plan;
+    if (!plan) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
+    int rc;
+    try {
+        rc = build_plan(ctx, op, items, n, *plan);
+    } catch (const std::bad_alloc &) {
+        rc = fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
+    }
+    if (rc != SFMM_OK) {
+        delete plan;
+        return rc;
+    }
+    plan->ctx = ctx;
+    plan->op = op;
+    *out = plan;
+    return SFMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm_plan **out)
 {
     if (!ctx || !out) return SFMM_ERR_LOGIC;
     std::lock_guard<std::mutex> lock(ctx->mu);
     *out = nullptr;
-    std::vector<sfmm_item> ordered;
-    std::vector<sfmm_plan::Launch> launches;
-    int rc = build_groups(ctx, op, items, n, ordered, launches);
+    sfmm_plan *plan = nullptr;
+    int rc = prepare_plan(ctx, op, items, n, &plan);
     if (rc != SFMM_OK) return rc;
     hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    sfmm_plan *plan = new (std::nothrow) sfmm_plan;
-    if (!plan) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
-    plan->ctx = ctx;
-    plan->op = op;
-    plan->n_items = ordered.size();
-    plan->launches = std::move(launches);
-    for (const sfmm_item &it : ordered) {
-        plan->src_extent = std::max<size_t>(plan->src_extent, it.src + size_t(it.p_in) * (it.p_in + 1));
-        plan->dst_extent = std::max<size_t>(plan->dst_extent, it.dst + size_t(it.p_out) * (it.p_out + 1));
-    }
-    if (!ordered.empty()) {
-        e = hipMalloc(&plan->d_items, ordered.size() * sizeof(sfmm_item));
+    if (e == hipSuccess && plan->dev_bytes) {
+        e = hipMalloc(&plan->d_mem, plan->dev_bytes);
         if (e != hipSuccess) {
+            plan->d_mem = nullptr;
             delete plan;
             return fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
         }
-        e = hipMemcpy(plan->d_items, ordered.data(), ordered.size() * sizeof(sfmm_item), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-            (void)hipFree(plan->d_items);
-            delete plan;
-            return hip_fail(ctx, e, "hipMemcpy");
-        }
+        e = hipMemcpy(plan->d_mem, plan->blob.data(), plan->blob.size(), hipMemcpyHostToDevice);
     }
+    if (e != hipSuccess) {
+        if (plan->d_mem) (void)hipFree(plan->d_mem);
+        delete plan;
+        return hip_fail(ctx, e, "plan upload");
+    }
+    plan->blob.clear();
+    plan->blob.shrink_to_fit();
     *out = plan;
+    return SFMM_OK;
+}
+
+int sfmm_plan_prepare(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm_plan **out)
+{
+    if (!ctx || !out) return SFMM_ERR_LOGIC;
+    *out = nullptr;
+    // build_plan reads only the context's order and kernel choices: copy
+    // them, so the preparation itself runs outside the lock
+    sfmm_context view;
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        view.device = ctx->device;
+        view.P = ctx->P;
+        view.family = ctx->family;
+        view.precision = ctx->precision;
+        std::memcpy(view.tune, ctx->tune, sizeof(view.tune));
+    }
+    sfmm_plan *plan = nullptr;
+    const int rc = prepare_plan(&view, op, items, n, &plan);
+    if (rc != SFMM_OK) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->err = view.err;
+        return rc;
+    }
+    plan->ctx = ctx;
+    plan->owns_mem = false;
+    *out = plan;
+    return SFMM_OK;
+}
+
+size_t sfmm_plan_device_bytes(const sfmm_plan *plan) { return plan ? plan->dev_bytes : 0; }
+
+int sfmm_plan_bind(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *stream)
+{
+    if (!plan) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = plan->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (plan->owns_mem) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan owns its device memory");
+    if (plan->dev_bytes && (!dev_mem || dev_bytes < plan->dev_bytes || (reinterpret_cast<uintptr_t>(dev_mem) & 255u)))
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: device buffer missing, misaligned or too small");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess && !plan->blob.empty())
+        e = hipMemcpyAsync(dev_mem, plan->blob.data(), plan->blob.size(), hipMemcpyHostToDevice,
+                           static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync");
+    plan->d_mem = static_cast<unsigned char *>(dev_mem);
     return SFMM_OK;
 }
 
@@ -272,7 +451,7 @@ void sfmm_plan_destroy(sfmm_plan *plan)
 {
     if (!plan) return;
     drop_graph(plan);
-    if (plan->d_items) (void)hipFree(plan->d_items);
+    if (plan->d_mem && plan->owns_mem) (void)hipFree(plan->d_mem);
     delete plan;
 }
 
@@ -303,6 +482,7 @@ int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, d
     sfmm_context *ctx = plan->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (plan->n_items && (!src_pool || !dst_pool)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pool");
+    if (plan->n_items && !plan->d_mem) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan not bound to device memory");
     if (src_len < plan->src_extent || dst_len < plan->dst_extent)
         return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
     hipError_t e = hipSetDevice(ctx->device);
@@ -318,6 +498,7 @@ int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, d
     sfmm_context *ctx = plan->ctx;
     std::lock_guard<std::mutex> lock(ctx->mu);
     if (plan->n_items && (!src_pool || !dst_pool)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pool");
+    if (plan->n_items && !plan->d_mem) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan not bound to device memory");
     if (src_len < plan->src_extent || dst_len < plan->dst_extent)
         return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
     hipError_t e = hipSetDevice(ctx->device);

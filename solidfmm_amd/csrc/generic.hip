@@ -58,7 +58,7 @@ __device__ void swap_g(const double *__restrict__ table, int n, bool tr, const d
 __global__ void __launch_bounds__(kThreads)
     k_translate_generic(int op, const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                         const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ table,
-                        const double *__restrict__ fac, const double *__restrict__ invfac, int pc)
+                        const double *__restrict__ fac, const double *__restrict__ invfac, int pc, int acc)
 {
     extern __shared__ double sh[];
     const int tri = pc * (pc + 1) / 2;
@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(kThreads)
         double x = it.x, y = it.y, z = it.z;
         const int pend = pi < po ? pi : po;
         if (op != SFMM_M2L && x * x + y * y + z * z == 0.0) {  // translations.cpp:539-547, 596-604
-            for (int i = tid; i < pend * (pend + 1); i += kThreads) D[i] += S[i];
+            for (int i = tid; i < pend * (pend + 1); i += kThreads) D[i] = (acc ? D[i] : 0.0) + S[i];
             continue;
         }
         if (op == SFMM_M2M) {
@@ -195,8 +195,10 @@ __global__ void __launch_bounds__(kThreads)
             const double scale = (op == SFMM_M2M) ? rp[N + 1] : rm[N];
             for (int m = tid; m <= N; m += kThreads) {
                 const double a = vbr[m], b = vbi[m];
-                D[N * (N + 1) + m] += scale * (ca[m] * a + sa[m] * b);
-                D[(N + 1) * (N + 1) + m] += scale * (ca[m] * b - sa[m] * a);
+                const double re = scale * (ca[m] * a + sa[m] * b);
+                const double im = scale * (ca[m] * b - sa[m] * a);
+                D[N * (N + 1) + m] = (acc ? D[N * (N + 1) + m] : 0.0) + re;
+                D[(N + 1) * (N + 1) + m] = (acc ? D[(N + 1) * (N + 1) + m] : 0.0) + im;
             }
             __syncthreads();
         }
@@ -212,7 +214,7 @@ size_t lds_bytes(int pc)
 }  // namespace generic
 
 hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
-                          hipStream_t stream)
+                          hipStream_t stream, int acc)
 {
     if (op < 0 || op > 2 || pc < 1 || pc > uint32_t(mfma::kMaxOrder)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
@@ -231,7 +233,7 @@ hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n,
     const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&generic::k_translate_generic),
                                           generic::kThreads, n);
     hipLaunchKernelGGL(generic::k_translate_generic, dim3(grid), dim3(generic::kThreads), lds, stream, op, src, dst,
-                       items, static_cast<uint32_t>(n), table, fac, fac + 2 * mfma::kMaxOrder, int(pc));
+                       items, static_cast<uint32_t>(n), table, fac, fac + 2 * mfma::kMaxOrder, int(pc), acc);
     return hipGetLastError();
 }
 

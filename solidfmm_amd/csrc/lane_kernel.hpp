@@ -255,18 +255,21 @@ __device__ __forceinline__ void backward(const Rot<PB> &R, double scale, double 
 }
 
 // Zero shift in M2M / L2L: target += source over the common degrees
-// (translations.cpp:539-547, 596-604).
-__device__ __forceinline__ void zero_shift_add(const double *__restrict__ S, double *__restrict__ D, uint32_t p)
+// (translations.cpp:539-547, 596-604); target = source when !acc.
+__device__ __forceinline__ void zero_shift_add(const double *__restrict__ S, double *__restrict__ D, uint32_t p,
+                                               bool acc)
 {
     const uint32_t len = p * (p + 1);
-    for (uint32_t i = 0; i < len; ++i) D[i] += S[i];
+    for (uint32_t i = 0; i < len; ++i) D[i] = (acc ? D[i] : 0.0) + S[i];
 }
 
-// One lane, one translation (see the head of this file for RT).
+// One lane, one translation (see the head of this file for RT).  acc:
+// target += T(source); !acc: target = T(source) over the degrees the
+// translation defines (scratch blocks of a segmented reduction).
 template <int OP, int PI, int PO, bool RT>
 __global__ void __launch_bounds__(64)
     k_translate(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
-                const sfmm_item *__restrict__ items, uint32_t n_items)
+                const sfmm_item *__restrict__ items, uint32_t n_items, int acc)
 {
     static_assert(!RT || PI == PO, "runtime orders use a square bound");
     constexpr int PM = PI > PO ? PI : PO;
@@ -286,7 +289,7 @@ __global__ void __launch_bounds__(64)
 
     if constexpr (OP != SFMM_M2L) {
         if (x * x + y * y + z * z == 0.0) {
-            zero_shift_add(S, D, RT ? pend : uint32_t(PE));
+            zero_shift_add(S, D, RT ? pend : uint32_t(PE), acc != 0);
             return;
         }
     }
@@ -299,6 +302,7 @@ __global__ void __launch_bounds__(64)
     const uint32_t pin_w = RT ? wave_max7(pin) : uint32_t(PI);
     const uint32_t pout_w = RT ? wave_max7(OP == SFMM_L2L ? pend : pout) : uint32_t(PO);
     auto out_on = [&](int N) { return !RT || uint32_t(N) < (OP == SFMM_L2L ? pend : pout); };
+    auto fetch_on = [&](int N) { return acc != 0 && out_on(N); };
 
     // Euler data (microkernel_generic.cpp:61-74).
     const double rho2 = x * x + y * y;
@@ -355,7 +359,7 @@ __global__ void __launch_bounds__(64)
         static_for<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             if constexpr (n == PI - 1) {
-                fetch_degree<PO - 1>(D, tg, out_on(PO - 1));
+                fetch_degree<PO - 1>(D, tg, fetch_on(PO - 1));
                 sched_fence();
             }
             if (!RT || uint32_t(n) < pin_w) {
@@ -381,7 +385,7 @@ __global__ void __launch_bounds__(64)
             constexpr int N = decltype(Nc)::value;
             double2 cur[PO];
             static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
-            if constexpr (N > 0) fetch_degree<N - 1>(D, tg, out_on(N - 1));
+            if constexpr (N > 0) fetch_degree<N - 1>(D, tg, fetch_on(N - 1));
             sched_fence();
             if (!RT || uint32_t(N) < pout_w) {
                 double re[N + 1], im[N + 1];
@@ -418,14 +422,14 @@ __global__ void __launch_bounds__(64)
         // M_N^m = sum_{K<=N} r^{N-K}/(N-K)! T_K^m (translations.cpp:363-377).
         double s = rinv;   // forward scale r^-(n+1)
         double sb = r;     // backward scale r^(N+1) (translations.cpp:129)
-        fetch_degree<0>(D, tg, out_on(0));
+        fetch_degree<0>(D, tg, fetch_on(0));
         sched_fence();
         static_for<0, PM>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             double2 cur[PO];
             if constexpr (n < PO) {
                 static_for<0, n + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
-                if constexpr (n + 1 < PO) fetch_degree<n + 1>(D, tg, out_on(n + 1));
+                if constexpr (n + 1 < PO) fetch_degree<n + 1>(D, tg, fetch_on(n + 1));
                 sched_fence();
             }
             if constexpr (n < PI) {
@@ -487,7 +491,7 @@ __global__ void __launch_bounds__(64)
             bs[n] = bs[n - 1] * rinv;
         });
         if constexpr (PE > 0) {
-            fetch_degree<PE - 1>(D, tg, out_on(PE - 1));
+            fetch_degree<PE - 1>(D, tg, fetch_on(PE - 1));
             sched_fence();
         }
         static_for_down<0, PI>([&](auto nc) {
@@ -504,7 +508,7 @@ __global__ void __launch_bounds__(64)
                 constexpr int N = n;
                 double2 cur[PO];
                 static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
-                if constexpr (N > 0) fetch_degree<N - 1>(D, tg, out_on(N - 1));
+                if constexpr (N > 0) fetch_degree<N - 1>(D, tg, fetch_on(N - 1));
                 sched_fence();
                 if (!RT || uint32_t(N) < pout_w) {
                     double re[N + 1], im[N + 1];
@@ -533,7 +537,7 @@ __global__ void __launch_bounds__(64)
     }
 }
 
-using kernel_fn = void (*)(const double *, double *, const sfmm_item *, uint32_t);
+using kernel_fn = void (*)(const double *, double *, const sfmm_item *, uint32_t, int);
 
 // Square orders 1..kSquare, all three operators.
 inline constexpr int kSquare = 12;

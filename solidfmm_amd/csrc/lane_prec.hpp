@@ -105,10 +105,10 @@ __device__ __forceinline__ void rot(const typename Pol::T *c, const typename Pol
 }
 
 template <int N, int CAP>
-__device__ __forceinline__ void fetch_degree(const double *__restrict__ blk, double2 (&v)[CAP])
+__device__ __forceinline__ void fetch_degree(const double *__restrict__ blk, double2 (&v)[CAP], bool on = true)
 {
     const double2 *q = reinterpret_cast<const double2 *>(blk + N * (N + 1));
-    static_for<0, N + 1>([&](auto jc) { v[decltype(jc)::value] = q[decltype(jc)::value]; });
+    static_for<0, N + 1>([&](auto jc) { v[decltype(jc)::value] = on ? q[decltype(jc)::value] : double2{0.0, 0.0}; });
 }
 
 template <int N, int CAP>
@@ -120,7 +120,7 @@ __device__ __forceinline__ double piece(const double2 (&v)[CAP], int k)
 template <class Pol, int P>
 __global__ void __launch_bounds__(64)
     k_m2l_prec(const double *__restrict__ src_pool, double *__restrict__ dst_pool, const sfmm_item *__restrict__ items,
-               uint32_t n_items)
+               uint32_t n_items, int acc)
 {
     using T = typename Pol::T;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(64)
         double2 cur[P];
         static_for<0, n + 1>([&](auto jc) { cur[decltype(jc)::value] = sv[decltype(jc)::value]; });
         if constexpr (n + 1 < P) fetch_degree<n + 1>(S, sv);
-        else fetch_degree<P - 1>(D, tg);
+        else fetch_degree<P - 1>(D, tg, acc != 0);
         sched_fence();
         T re[n + 1], im[n + 1];
         static_for<0, n + 1>([&](auto mc) {
@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(64)
         constexpr int N = decltype(Nc)::value;
         double2 old[P];
         static_for<0, N + 1>([&](auto jc) { old[decltype(jc)::value] = tg[decltype(jc)::value]; });
-        if constexpr (N > 0) fetch_degree<N - 1>(D, tg);
+        if constexpr (N > 0) fetch_degree<N - 1>(D, tg, acc != 0);
         sched_fence();
         T re[N + 1], im[N + 1];
         static_for<0, N + 1>([&](auto mc) {
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(64)
     });
 }
 
-using kernel_fn = void (*)(const double *, double *, const sfmm_item *, uint32_t);
+using kernel_fn = void (*)(const double *, double *, const sfmm_item *, uint32_t, int);
 
 template <class Pol, int... Ps>
 constexpr auto make_table(std::integer_sequence<int, Ps...>)

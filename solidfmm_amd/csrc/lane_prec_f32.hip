@@ -14,7 +14,7 @@ kernel_fn kernel_f32(uint32_t P)
 }  // namespace lprec
 
 hipError_t launch_prec_lane_m2l(int precision, uint32_t P, const sfmm_item *items, size_t n, const double *src,
-                                double *dst, hipStream_t stream)
+                                double *dst, hipStream_t stream, int acc)
 {
     if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
     lprec::kernel_fn f = nullptr;
@@ -27,7 +27,7 @@ hipError_t launch_prec_lane_m2l(int precision, uint32_t P, const sfmm_item *item
     if (n == 0) return hipSuccess;
     const uint32_t block = 64;
     const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
-    hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n));
+    hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n), acc);
     return hipGetLastError();
 }
 

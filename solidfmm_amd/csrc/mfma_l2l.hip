@@ -6,7 +6,7 @@ namespace sfmm {
 namespace mfma {
 
 template hipError_t launch_op<SFMM_L2L, false>(uint32_t, const sfmm_item *, size_t, const double *, double *,
-                                                   const double *, hipStream_t);
+                                                   const double *, hipStream_t, int);
 
 }  // namespace mfma
 }  // namespace sfmm

@@ -83,7 +83,7 @@ template <class Pol>
 __global__ void __launch_bounds__(kThreads)
     k_translate_prec(int op, const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                      const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ table,
-                     const double *__restrict__ fac, const double *__restrict__ invfac, int pc)
+                     const double *__restrict__ fac, const double *__restrict__ invfac, int pc, int acc)
 {
     using T = typename Pol::T;
     extern __shared__ double sh_raw[];
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(kThreads)
         double x = it.x, y = it.y, z = it.z;
         const int pend = pi < po ? pi : po;
         if (op != SFMM_M2L && x * x + y * y + z * z == 0.0) {  // translations.cpp:539-547, 596-604
-            for (int i = tid; i < pend * (pend + 1); i += kThreads) D[i] += S[i];
+            for (int i = tid; i < pend * (pend + 1); i += kThreads) D[i] = (acc ? D[i] : 0.0) + S[i];
             continue;
         }
         if (op == SFMM_M2M) {
@@ -251,8 +251,10 @@ __global__ void __launch_bounds__(kThreads)
                 const T re = Pol::add(Pol::mul(ca[m], a), Pol::mul(sa[m], b));
                 const T im = Pol::sub(Pol::mul(ca[m], b), Pol::mul(sa[m], a));
                 const int q = -qside(ltype_bwd, N, m);
-                D[N * (N + 1) + m] += scale * ldexp(Pol::to(re), q);
-                D[(N + 1) * (N + 1) + m] += scale * ldexp(Pol::to(im), q);
+                const double vr = scale * ldexp(Pol::to(re), q);
+                const double vi = scale * ldexp(Pol::to(im), q);
+                D[N * (N + 1) + m] = (acc ? D[N * (N + 1) + m] : 0.0) + vr;
+                D[(N + 1) * (N + 1) + m] = (acc ? D[(N + 1) * (N + 1) + m] : 0.0) + vi;
             }
             __syncthreads();
         }
@@ -268,7 +270,7 @@ size_t lds_bytes(int pc)
 
 template <class Pol>
 hipError_t launch(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
-                  hipStream_t stream)
+                  hipStream_t stream, int acc)
 {
     const double *table = mfma::device_table();
     const double *fac = mfma::device_factorials();
@@ -276,21 +278,21 @@ hipError_t launch(int op, uint32_t pc, const sfmm_item *items, size_t n, const d
     const size_t lds = lds_bytes(int(pc));
     const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_translate_prec<Pol>), kThreads, n);
     hipLaunchKernelGGL(k_translate_prec<Pol>, dim3(grid), dim3(kThreads), lds, stream, op, src, dst, items,
-                       static_cast<uint32_t>(n), table, fac, fac + 2 * mfma::kMaxOrder, int(pc));
+                       static_cast<uint32_t>(n), table, fac, fac + 2 * mfma::kMaxOrder, int(pc), acc);
     return hipGetLastError();
 }
 
 }  // namespace precgen
 
 hipError_t launch_prec_generic(int precision, int op, uint32_t pc, const sfmm_item *items, size_t n,
-                               const double *src, double *dst, hipStream_t stream)
+                               const double *src, double *dst, hipStream_t stream, int acc)
 {
     if (op < 0 || op > 2 || pc < 1 || pc > kPrecMaxOrder) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
     switch (precision) {
-    case SFMM_FP32: return precgen::launch<prec::F32>(op, pc, items, n, src, dst, stream);
-    case SFMM_SPLIT32: return precgen::launch<prec::DW32>(op, pc, items, n, src, dst, stream);
+    case SFMM_FP32: return precgen::launch<prec::F32>(op, pc, items, n, src, dst, stream, acc);
+    case SFMM_SPLIT32: return precgen::launch<prec::DW32>(op, pc, items, n, src, dst, stream, acc);
     default: return hipErrorInvalidValue;
     }
 }

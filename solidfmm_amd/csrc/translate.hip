@@ -78,12 +78,12 @@ namespace {
 
 template <bool RT>
 hipError_t launch_mfma(int op, uint32_t pk, const sfmm_item *items, size_t n, const double *src, double *dst,
-                       const double *table, hipStream_t stream)
+                       const double *table, hipStream_t stream, int acc)
 {
     switch (op) {
-    case SFMM_M2M: return mfma::launch_op<SFMM_M2M, RT>(pk, items, n, src, dst, table, stream);
-    case SFMM_M2L: return mfma::launch_op<SFMM_M2L, RT>(pk, items, n, src, dst, table, stream);
-    default: return mfma::launch_op<SFMM_L2L, RT>(pk, items, n, src, dst, table, stream);
+    case SFMM_M2M: return mfma::launch_op<SFMM_M2M, RT>(pk, items, n, src, dst, table, stream, acc);
+    case SFMM_M2L: return mfma::launch_op<SFMM_M2L, RT>(pk, items, n, src, dst, table, stream, acc);
+    default: return mfma::launch_op<SFMM_L2L, RT>(pk, items, n, src, dst, table, stream, acc);
     }
 }
 
@@ -92,10 +92,12 @@ hipError_t launch_mfma(int op, uint32_t pk, const sfmm_item *items, size_t n, co
 hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, size_t n, const double *src,
                          double *dst, hipStream_t stream)
 {
+    const int acc = k.acc ? 1 : 0;
     if (k.precision != SFMM_FP64) {
         if (k.family == kLaneKernels && op == SFMM_M2L && !k.rt)
-            return launch_prec_lane_m2l(k.precision, k.pk, items, n, src, dst, stream);
-        if (k.family == kGenericKernel) return launch_prec_generic(k.precision, op, k.pk, items, n, src, dst, stream);
+            return launch_prec_lane_m2l(k.precision, k.pk, items, n, src, dst, stream, acc);
+        if (k.family == kGenericKernel)
+            return launch_prec_generic(k.precision, op, k.pk, items, n, src, dst, stream, acc);
         return hipErrorInvalidValue;
     }
     if (op < 0 || op > 2 || !family_covers(k.family, k.pk)) return hipErrorInvalidValue;
@@ -107,16 +109,16 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
         lane::kernel_fn f = t.f[op][k.pk - 1];
         const uint32_t block = 64;
         const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
-        hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n));
+        hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n), acc);
         return hipGetLastError();
     }
     case kMfmaKernels: {
         const double *table = mfma::device_table();
         if (!table) return hipErrorOutOfMemory;
-        return k.rt ? launch_mfma<true>(op, k.pk, items, n, src, dst, table, stream)
-                    : launch_mfma<false>(op, k.pk, items, n, src, dst, table, stream);
+        return k.rt ? launch_mfma<true>(op, k.pk, items, n, src, dst, table, stream, acc)
+                    : launch_mfma<false>(op, k.pk, items, n, src, dst, table, stream, acc);
     }
-    default: return launch_generic(op, k.pk, items, n, src, dst, stream);
+    default: return launch_generic(op, k.pk, items, n, src, dst, stream, acc);
     }
 }
 

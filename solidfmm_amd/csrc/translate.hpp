@@ -29,7 +29,7 @@ inline constexpr int kMfmaMin = 1;
 inline constexpr int kMfmaMax = 24;
 template <int OP, bool RT>
 hipError_t launch_op(uint32_t P, const sfmm_item *items, size_t n, const double *src, double *dst,
-                     const double *table, hipStream_t stream);
+                     const double *table, hipStream_t stream, int acc);
 }  // namespace mfma
 
 // Highest order with a kernel.
@@ -56,20 +56,40 @@ inline constexpr uint32_t kPrecLaneMax = 12;    // FP32
 inline constexpr uint32_t kPrecLaneMaxDW = 10;  // double-word FP32
 inline constexpr uint32_t kPrecMaxOrder = 24;
 hipError_t launch_prec_lane_m2l(int precision, uint32_t P, const sfmm_item *items, size_t n, const double *src,
-                                double *dst, hipStream_t stream);
+                                double *dst, hipStream_t stream, int acc);
 hipError_t launch_prec_generic(int precision, int op, uint32_t pc, const sfmm_item *items, size_t n,
-                               const double *src, double *dst, hipStream_t stream);
+                               const double *src, double *dst, hipStream_t stream, int acc);
 
 // One launch of a plan: family (not kAuto), the kernel's order bound pk,
-// whether items carry their own orders (rt), and the arithmetic
-// (SFMM_FP64, SFMM_FP32, SFMM_SPLIT32).  The generic kernels always read
-// per-item orders; pk is then the largest order of the group.
+// whether items carry their own orders (rt), the arithmetic (SFMM_FP64,
+// SFMM_FP32, SFMM_SPLIT32) and whether targets accumulate (acc) or are
+// overwritten with the translation (scratch blocks of a segmented
+// reduction).  The generic kernels always read per-item orders; pk is then
+// the largest order of the group.
 struct KernelChoice {
     int family;
     uint32_t pk;
     bool rt;
     int precision = SFMM_FP64;
+    bool acc = true;
 };
+
+// Segmented reduction: target block t += sum of its items' scratch blocks,
+// in list order.  Both arrays are device memory.
+struct RedTarget {
+    uint64_t dst;    // offset (reals) of the target block in the target pool
+    uint32_t first;  // first entry of the target's items in RedItem[]
+    uint32_t count;  // number of items
+    uint32_t len;    // reals to add (the longest of the items' blocks)
+    uint32_t pad;
+};
+struct RedItem {
+    uint64_t off;  // offset (reals) of the item's block in the scratch pool
+    uint32_t len;  // reals the translation wrote
+    uint32_t pad;
+};
+hipError_t launch_reduce(const RedTarget *targets, size_t n_targets, const RedItem *items, const double *scratch,
+                         double *dst, hipStream_t stream);
 
 // Enqueue dst += T(src) for n device-resident items with pairwise distinct
 // targets and orders within `k`.
@@ -78,7 +98,7 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
 
 // The generic kernel: items of any orders up to `pc`.
 hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
-                          hipStream_t stream);
+                          hipStream_t stream, int acc = 1);
 
 // Enqueue dst += T(src) for n items all of orders (p_in, p_out) with
 // pairwise distinct targets, in `family` (kAuto: default_family).

@@ -200,7 +200,7 @@ __device__ __forceinline__ uint32_t wave_max7(uint32_t v)
 template <int OP, int PI, int PO, bool RT>
 __device__ __forceinline__ void translate_group(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                                                 const sfmm_item *__restrict__ items, uint32_t n_items, uint32_t g,
-                                                const double *tab, int lane)
+                                                const double *tab, int lane, bool acc)
 {
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int PE = PI < PO ? PI : PO;
@@ -330,7 +330,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             oi[s] = 0.0;
             if constexpr (4 * s <= N) {
                 const int m = q + 4 * s;
-                if (active && m <= N && on) {
+                if (active && m <= N && on && acc) {
                     orr[s] = D[N * (N + 1) + m];
                     oi[s] = D[(N + 1) * (N + 1) + m];
                 }
@@ -547,7 +547,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
         // degrees (translations.cpp:539-547, 596-604); quarter-wave q takes
         // every fourth real.
         const int len = RT ? int(pend * (pend + 1)) : PE * (PE + 1);
-        for (int i = q; i < len; i += 4) D[i] += S[i];
+        for (int i = q; i < len; i += 4) D[i] = (acc ? D[i] : 0.0) + S[i];
     }
 }
 
@@ -556,7 +556,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
 template <int OP, int PI, int PO, bool RT>
 __global__ void __launch_bounds__(64 * kWaves)
     k_translate_mfma(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
-                     const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ table)
+                     const sfmm_item *__restrict__ items, uint32_t n_items, const double *__restrict__ table, int acc)
 {
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int TAB = dense_size(PM);
@@ -568,10 +568,10 @@ __global__ void __launch_bounds__(64 * kWaves)
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t groups = (n_items + 15u) / 16u;
     for (uint32_t g = blockIdx.x * kWaves + wave; g < groups; g += gridDim.x * kWaves)
-        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab, lane);
+        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab, lane, acc != 0);
 }
 
-using fn_t = void (*)(const double *, double *, const sfmm_item *, uint32_t, const double *);
+using fn_t = void (*)(const double *, double *, const sfmm_item *, uint32_t, const double *, int);
 
 template <int OP, bool RT, int... Ps>
 constexpr auto make_table(std::integer_sequence<int, Ps...>)
@@ -587,7 +587,7 @@ constexpr auto make_table(std::integer_sequence<int, Ps...>)
 // (mfma_{m2m,m2l,l2l}{,_rt}.hip) so they compile in parallel.
 template <int OP, bool RT>
 hipError_t launch_op(uint32_t P, const sfmm_item *items, size_t n, const double *src, double *dst,
-                     const double *table, hipStream_t stream)
+                     const double *table, hipStream_t stream, int acc)
 {
     static const auto tab = make_table<OP, RT>(std::make_integer_sequence<int, kMfmaMax - kMfmaMin + 1>{});
     if (P < uint32_t(kMfmaMin) || P > uint32_t(kMfmaMax)) return hipErrorInvalidValue;
@@ -596,7 +596,7 @@ hipError_t launch_op(uint32_t P, const sfmm_item *items, size_t n, const double 
     const uint32_t grid =
         persistent_grid(reinterpret_cast<const void *>(f), 64 * kWaves, (groups + kWaves - 1) / kWaves);
     hipLaunchKernelGGL(f, dim3(grid), dim3(64 * kWaves), 0, stream, src, dst, items, static_cast<uint32_t>(n),
-                       table);
+                       table, acc);
     return hipGetLastError();
 }
 

@@ -142,3 +142,43 @@ def test_mixed_full_size_sample(sfmm, orc):
     assert_mixed_parity(got, want, sub, label="sample")
     assert np.isfinite(got).all()
     assert plan.launches <= 17
+
+
+@pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
+def test_heavily_repeated_targets(sfmm, orc, op):
+    """Targets hit hundreds of times (as in FMM interaction lists): per-item
+    scratch blocks and a segmented reduction in list order."""
+    import oracle
+    n, nt, P = 3000, 9, 14
+    items, src, _ = W.mixed_batch(n, 0xAB + op, 4, P, op=op)
+    items["p_out"] = P  # one order per target solid
+    blk = P * (P + 1)
+    tgt = (W.uniform01(0x55, op, n) * nt).astype(np.int64)
+    items["dst"] = (tgt * blk).astype(np.uint64)
+    dst0 = W.uniform(0x66, op, nt * blk, -1.0, 1.0)
+    got, plan = run(sfmm, op, items, src, dst0, P)
+    want = orc.translate(op, items, src, dst0.copy())
+    # each target sums ~330 contributions, added in another order than the
+    # oracle's one-by-one accumulation
+    e = oracle.degree_errors(got, want, P).max(axis=1)
+    assert e.max() <= 1e-11, e.max()
+    again, _ = run(sfmm, op, items, src, dst0, P)
+    assert np.array_equal(again, got)  # bitwise reproducible
+
+
+def test_prepared_plan_matches_created(sfmm):
+    items, src, _ = W.mixed_batch(4000, 0x99, 4, 16)
+    items["dst"] = (np.arange(4000) % 50 * 272).astype(np.uint64)  # 80 items per target
+    ctx = sfmm.Context(16)
+    s = dev(src)
+    a = torch.zeros(50 * 272, dtype=torch.float64, device="cuda")
+    b = torch.zeros_like(a)
+    sfmm.Plan(ctx, W.M2L, items).execute(s, a)
+    p = sfmm.Plan(ctx, W.M2L, items, prepared=True)
+    assert p.device_bytes > 4000 * 48  # reduction tables and scratch included
+    buf = torch.empty(p.device_bytes, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    p.bind(buf.data_ptr(), p.device_bytes, stream)
+    p.execute(s, b, stream)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)

@@ -1,0 +1,103 @@
+"""Continuous-batching request loop on the GPU: every request's result is
+the same alone and in the mix -- needs an MI355X.
+
+The FMM passes of a small Plummer octree (orders 20 at the root to 6 at
+the leaves) are cut into requests.  "Alone": each request runs by itself,
+synchronised, in list order.  "Mix": the request loop admits them into 4
+concurrent streams under a small item budget, in arrival order, with the
+hazard rules of solidfmm_amd.serving.  Both must leave identical pools (the
+same kernels see the same updates per cell in the same order), and the
+alone run must match the oracle translating the same lists.
+"""
+import numpy as np
+import pytest
+
+from solidfmm_amd import octree as O
+from solidfmm_amd import serving as S
+from solidfmm_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def sfmm():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a visible GPU")
+    import solidfmm_amd
+    solidfmm_amd.load()
+    return solidfmm_amd
+
+
+def workload(npts, seed, lo, hi):
+    tree = O.Octree(O.plummer(npts, seed), leaf_size=64)
+    P, off, pool_len, passes = O.translation_lists(tree, 20, 6)
+    m0 = W.uniform(seed, 7, pool_len, -1.0, 1.0)
+    for c in range(len(tree)):  # Im C_n^0 = 0
+        p = int(P[c])
+        m0[int(off[c]) + (np.arange(p) + 1) ** 2] = 0.0
+    reqs = S.make_requests(passes, seed, lo=lo, hi=hi)
+    return tree, pool_len, m0, reqs
+
+
+def pools_from(m0, pool_len):
+    return (torch.from_numpy(m0.copy()).cuda(), torch.zeros(pool_len, dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_alone_equals_mix(sfmm, precision):
+    tree, pool_len, m0, reqs = workload(20000, 5, 300, 3000)
+    ctx = sfmm.Context(20, precision=precision)
+    alone = pools_from(m0, pool_len)
+    S.run_sequential(ctx, alone, reqs)
+    torch.cuda.synchronize()
+    mix = pools_from(m0, pool_len)
+    budget = 8 << 20  # a few requests at a time (M2L requests carry scratch)
+    server = S.Server(ctx, mix, len(tree), slots=4, budget_bytes=budget)
+    stats = server.run(reqs, realtime=False)
+    print(f"precision={precision} requests={len(reqs)} stats={stats}", flush=True)
+    for a, b in zip(alone, mix):
+        assert torch.equal(a, b)
+    assert stats["max_inflight"] >= 2
+    assert stats["arena_peak_bytes"] <= budget
+    assert all(r.done is not None for r in reqs)
+
+
+def test_alone_matches_oracle(sfmm, orc):
+    tree, pool_len, m0, reqs = workload(4000, 6, 200, 1000)
+    ctx = sfmm.Context(20)
+    pools = pools_from(m0, pool_len)
+    S.run_sequential(ctx, pools, reqs)
+    got_m, got_l = (p.cpu().numpy() for p in pools)
+    m, l = m0.copy(), np.zeros(pool_len)
+    for r in reqs:
+        src = m if r.op in (W.M2M, W.M2L) else l
+        dst = m if r.op == W.M2M else l
+        orc.translate(r.op, r.items, src.copy() if src is dst else src, dst)
+    P = tree.orders(20, 6)
+    off = np.concatenate([[0], np.cumsum(P * (P + 1))[:-1]])
+    import oracle
+    for got, want in ((got_m, m), (got_l, l)):
+        worst = 0.0
+        for c in range(len(tree)):
+            p = int(P[c])
+            blk = slice(int(off[c]), int(off[c]) + p * (p + 1))
+            if not np.any(want[blk]):
+                continue
+            e = oracle.degree_errors(got[blk][None], want[blk][None], p).max()
+            worst = max(worst, e)
+        # sums over many translations of orders up to 20: FP64 tolerance of
+        # order 20 (tests/helpers.tol_fp64) with the 10x max headroom
+        assert worst <= 2e-8, worst
+
+
+def test_poisson_run_reports(sfmm):
+    tree, pool_len, m0, reqs = workload(20000, 8, 300, 3000)
+    ctx = sfmm.Context(20)
+    S.poisson_arrivals(reqs, 2000.0, 3)
+    server = S.Server(ctx, pools_from(m0, pool_len), len(tree), slots=4)
+    st = server.run(reqs, realtime=True)
+    assert st["translations"] == sum(len(r.items) for r in reqs)
+    assert 0 < st["latency_ms"]["p50"] <= st["latency_ms"]["p99"] <= st["latency_ms"]["max"]
+    assert st["translations_per_s"] > 0
