@@ -44,34 +44,12 @@ namespace lane {
 
 constexpr int tri(int m, int n) { return n * (n + 1) / 2 + m; }
 
-template <int PB>
+// e^{i alpha} and e^{i beta}.  The powers e^{i m theta} are formed inside
+// each rotation by the reference's recurrence (microkernel_generic.cpp:
+// 76-96), so no table of powers occupies registers.
 struct Rot {
-    double ca[PB], sa[PB], cb[PB], sb[PB];
+    double ca, sa, cb, sb;
 };
-
-// Powers e^{i k alpha}, e^{i k beta} by repeated complex products
-// (microkernel_generic.cpp:76-96).
-template <int PB>
-__device__ __forceinline__ void euler_powers(double ca1, double sa1, double cb1, double sb1, Rot<PB> &R)
-{
-    R.ca[0] = 1.0;
-    R.sa[0] = 0.0;
-    R.cb[0] = 1.0;
-    R.sb[0] = 0.0;
-    if constexpr (PB > 1) {
-        R.ca[1] = ca1;
-        R.sa[1] = sa1;
-        R.cb[1] = cb1;
-        R.sb[1] = sb1;
-    }
-    static_for<2, PB>([&](auto ic_) {
-        constexpr int i = decltype(ic_)::value;
-        R.ca[i] = R.ca[i - 1] * ca1 - R.sa[i - 1] * sa1;
-        R.sa[i] = R.sa[i - 1] * ca1 + R.ca[i - 1] * sa1;
-        R.cb[i] = R.cb[i - 1] * cb1 - R.sb[i - 1] * sb1;
-        R.sb[i] = R.sb[i - 1] * cb1 + R.cb[i - 1] * sb1;
-    });
-}
 
 // out[m] = sum_l A[m][l] in[l] for the folded real (KR) and imaginary (KI)
 // blocks of degree N (microkernel_generic.cpp:142-214).
@@ -107,55 +85,35 @@ __device__ __forceinline__ void swap_stage(double (&re)[N + 1], double (&im)[N +
     });
 }
 
-// (re,im) <- scale * e^{+i m theta} (re,im)      (rotscale, forward=true)
-template <int N>
-__device__ __forceinline__ void rot_fwd(const double *c, const double *s, double scale, double (&re)[N + 1],
-                                        double (&im)[N + 1])
+// (re,im) <- scale * e^{+i m theta} (re,im) (FWD, rotscale forward=true)
+// or scale * e^{-i m theta} (re,im); e^{i m theta} = (c1 + i s1)^m built by
+// repeated products on the way.
+template <int N, bool FWD, bool SCALE>
+__device__ __forceinline__ void rot(double c1, double s1, double scale, double (&re)[N + 1], double (&im)[N + 1])
 {
-    re[0] *= scale;
-    im[0] *= scale;
+    if constexpr (SCALE) {
+        re[0] *= scale;
+        im[0] *= scale;
+    }
+    double c = c1, s = s1;
     static_for<1, N + 1>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
+        if constexpr (m > 1) {
+            const double c2 = c * c1 - s * s1;
+            s = s * c1 + c * s1;
+            c = c2;
+        }
         const double a = re[m], b = im[m];
-        re[m] = scale * (c[m] * a - s[m] * b);
-        im[m] = scale * (s[m] * a + c[m] * b);
-    });
-}
-
-template <int N>
-__device__ __forceinline__ void rot_fwd1(const double *c, const double *s, double (&re)[N + 1], double (&im)[N + 1])
-{
-    static_for<1, N + 1>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const double a = re[m], b = im[m];
-        re[m] = c[m] * a - s[m] * b;
-        im[m] = s[m] * a + c[m] * b;
-    });
-}
-
-// (re,im) <- scale * e^{-i m theta} (re,im)      (rotscale, forward=false)
-template <int N>
-__device__ __forceinline__ void rot_bwd(const double *c, const double *s, double scale, double (&re)[N + 1],
-                                        double (&im)[N + 1])
-{
-    re[0] *= scale;
-    im[0] *= scale;
-    static_for<1, N + 1>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const double a = re[m], b = im[m];
-        re[m] = scale * (c[m] * a + s[m] * b);
-        im[m] = scale * (c[m] * b - s[m] * a);
-    });
-}
-
-template <int N>
-__device__ __forceinline__ void rot_bwd1(const double *c, const double *s, double (&re)[N + 1], double (&im)[N + 1])
-{
-    static_for<1, N + 1>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        const double a = re[m], b = im[m];
-        re[m] = c[m] * a + s[m] * b;
-        im[m] = c[m] * b - s[m] * a;
+        double u, v;
+        if constexpr (FWD) {
+            u = c * a - s * b;
+            v = s * a + c * b;
+        } else {
+            u = c * a + s * b;
+            v = c * b - s * a;
+        }
+        re[m] = SCALE ? scale * u : u;
+        im[m] = SCALE ? scale * v : v;
     });
 }
 
@@ -230,29 +188,35 @@ __device__ __forceinline__ uint32_t wave_max7(uint32_t v)
 
 // Forward pipelines: M-type (M2M, M2L; translations.cpp:36-85) uses the
 // folded blocks, L-type (L2L; translations.cpp:189-239) their transposes.
-template <int N, bool LTYPE, int PB>
-__device__ __forceinline__ void forward(const Rot<PB> &R, double scale, double (&re)[N + 1], double (&im)[N + 1])
+template <int N, bool LTYPE>
+__device__ __forceinline__ void forward(const Rot &R, double scale, double (&re)[N + 1], double (&im)[N + 1])
 {
     constexpr int KR = LTYPE ? 2 : 0;
     constexpr int KI = LTYPE ? 3 : 1;
-    rot_fwd<N>(R.ca, R.sa, scale, re, im);
+    rot<N, true, true>(R.ca, R.sa, scale, re, im);
     swap_stage<N, KR, KI>(re, im);
-    rot_fwd1<N>(R.cb, R.sb, re, im);
+    rot<N, true, false>(R.cb, R.sb, 1.0, re, im);
     swap_stage<N, KR, KI>(re, im);
 }
 
 // Backward pipelines: M-type (M2M; translations.cpp:87-136), L-type
 // (M2L, L2L; translations.cpp:138-187).
-template <int N, bool LTYPE, int PB>
-__device__ __forceinline__ void backward(const Rot<PB> &R, double scale, double (&re)[N + 1], double (&im)[N + 1])
+template <int N, bool LTYPE>
+__device__ __forceinline__ void backward(const Rot &R, double scale, double (&re)[N + 1], double (&im)[N + 1])
 {
     constexpr int KR = LTYPE ? 2 : 0;
     constexpr int KI = LTYPE ? 3 : 1;
     swap_stage<N, KR, KI>(re, im);
-    rot_bwd1<N>(R.cb, R.sb, re, im);
+    rot<N, false, false>(R.cb, R.sb, 1.0, re, im);
     swap_stage<N, KR, KI>(re, im);
-    rot_bwd<N>(R.ca, R.sa, scale, re, im);
+    rot<N, false, true>(R.ca, R.sa, scale, re, im);
 }
+
+// Zeros read in place of a target that is not accumulated (acc == 0, or an
+// output degree a per-item-order lane does not store): loads are never
+// skipped, so no branch splits the load stream and the compiler can wait
+// for each source piece separately.
+inline __device__ double kZeroBlock[kMaxOrder * (kMaxOrder + 1)];
 
 // Zero shift in M2M / L2L: target += source over the common degrees
 // (translations.cpp:539-547, 596-604); target = source when !acc.
@@ -302,7 +266,9 @@ __global__ void __launch_bounds__(64)
     const uint32_t pin_w = RT ? wave_max7(pin) : uint32_t(PI);
     const uint32_t pout_w = RT ? wave_max7(OP == SFMM_L2L ? pend : pout) : uint32_t(PO);
     auto out_on = [&](int N) { return !RT || uint32_t(N) < (OP == SFMM_L2L ? pend : pout); };
-    auto fetch_on = [&](int N) { return acc != 0 && out_on(N); };
+    // where degree N's old target values come from: the target, or zeros
+    const double *__restrict__ Dr = acc ? D : kZeroBlock;
+    auto read_from = [&](int N) { return out_on(N) ? Dr : kZeroBlock; };
 
     // Euler data (microkernel_generic.cpp:61-74).
     const double rho2 = x * x + y * y;
@@ -315,9 +281,7 @@ __global__ void __launch_bounds__(64)
         ca1 = y * q;
         sa1 = x * q;
     }
-    const double cb1 = z * rinv, sb1 = -rho * rinv;
-    Rot<PM> R;
-    euler_powers<PM>(ca1, sa1, cb1, sb1, R);
+    const Rot R{ca1, sa1, z * rinv, -rho * rinv};
 
     // Whole source block in flight at once.  Its registers are handed over
     // degree by degree to the rotated triangle, so this costs no peak.
@@ -330,9 +294,19 @@ __global__ void __launch_bounds__(64)
             sv[j] = (!RT || uint32_t(j) < have) ? q[j] : double2{0.0, 0.0};
         });
     }
-    // Target degrees are prefetched one ahead of their use (tg); the
-    // scheduling fences keep those loads where they are issued.
-    double2 tg[PO];
+    // Target degrees are prefetched kAhead degrees ahead of their use, into
+    // a ring of buffers (slot = degree mod kAhead); the scheduling fences
+    // keep those loads where they are issued.
+    constexpr int kAhead = 1;
+    double2 tg[kAhead][PO];
+    auto take = [&](auto Nc, double2(&cur)[PO]) {
+        constexpr int N = decltype(Nc)::value;
+        static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[N % kAhead][decltype(jc)::value]; });
+    };
+    auto prefetch = [&](auto Nc) {
+        constexpr int N = decltype(Nc)::value;
+        if constexpr (N >= 0 && N < PO) fetch_degree<N>(read_from(N), tg[N % kAhead], true);
+    };
 
     // Rotated triangle: T[tri(m,K)] holds order m of degree K.
     double tre[tri(0, PI)], tim[tri(0, PI)];
@@ -359,17 +333,18 @@ __global__ void __launch_bounds__(64)
         static_for<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             if constexpr (n == PI - 1) {
-                fetch_degree<PO - 1>(D, tg, fetch_on(PO - 1));
+                static_for<0, kAhead>([&](auto dc) { prefetch(ic<PO - 1 - decltype(dc)::value>{}); });
                 sched_fence();
             }
             if (!RT || uint32_t(n) < pin_w) {
                 double re[n + 1], im[n + 1];
                 unpack_block_degree<n>(sv, re, im);
-                forward<n, false, PM>(R, s, re, im);
+                forward<n, false>(R, s, re, im);
                 keep(nc, re, im);
             } else {
                 clear(nc);
             }
+            sched_fence();
             s *= rinv;
         });
         double rp[PO];  // r^-N, the backward scale (translations.cpp:180)
@@ -384,8 +359,8 @@ __global__ void __launch_bounds__(64)
         static_for_down<0, PO>([&](auto Nc) {
             constexpr int N = decltype(Nc)::value;
             double2 cur[PO];
-            static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
-            if constexpr (N > 0) fetch_degree<N - 1>(D, tg, fetch_on(N - 1));
+            take(Nc, cur);
+            prefetch(ic<N - kAhead>{});
             sched_fence();
             if (!RT || uint32_t(N) < pout_w) {
                 double re[N + 1], im[N + 1];
@@ -412,7 +387,7 @@ __global__ void __launch_bounds__(64)
                     re[m] = ar;
                     im[m] = ai;
                 });
-                backward<N, true, PM>(R, rp[N], re, im);
+                backward<N, true>(R, rp[N], re, im);
                 add_store_degree<N>(D, cur, re, im, out_on(N));
             }
         });
@@ -422,25 +397,26 @@ __global__ void __launch_bounds__(64)
         // M_N^m = sum_{K<=N} r^{N-K}/(N-K)! T_K^m (translations.cpp:363-377).
         double s = rinv;   // forward scale r^-(n+1)
         double sb = r;     // backward scale r^(N+1) (translations.cpp:129)
-        fetch_degree<0>(D, tg, fetch_on(0));
+        static_for<0, kAhead>([&](auto dc) { prefetch(ic<decltype(dc)::value>{}); });
         sched_fence();
         static_for<0, PM>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             double2 cur[PO];
             if constexpr (n < PO) {
-                static_for<0, n + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
-                if constexpr (n + 1 < PO) fetch_degree<n + 1>(D, tg, fetch_on(n + 1));
+                take(nc, cur);
+                prefetch(ic<n + kAhead>{});
                 sched_fence();
             }
             if constexpr (n < PI) {
                 if (!RT || uint32_t(n) < pin_w) {
                     double re[n + 1], im[n + 1];
                     unpack_block_degree<n>(sv, re, im);
-                    forward<n, false, PM>(R, s, re, im);
+                    forward<n, false>(R, s, re, im);
                     keep(nc, re, im);
                 } else {
                     clear(nc);
                 }
+                sched_fence();
                 s *= rinv;
             }
             if constexpr (n < PO) {
@@ -467,7 +443,7 @@ __global__ void __launch_bounds__(64)
                         re[m] = ar;
                         im[m] = ai;
                     });
-                    backward<N, false, PM>(R, sb, re, im);
+                    backward<N, false>(R, sb, re, im);
                     add_store_degree<N>(D, cur, re, im, out_on(N));
                 }
                 sb *= r;
@@ -490,25 +466,24 @@ __global__ void __launch_bounds__(64)
             constexpr int n = decltype(nc)::value;
             bs[n] = bs[n - 1] * rinv;
         });
-        if constexpr (PE > 0) {
-            fetch_degree<PE - 1>(D, tg, fetch_on(PE - 1));
-            sched_fence();
-        }
+        static_for<0, kAhead>([&](auto dc) { prefetch(ic<PE - 1 - decltype(dc)::value>{}); });
+        sched_fence();
         static_for_down<0, PI>([&](auto nc) {
             constexpr int n = decltype(nc)::value;
             if (!RT || uint32_t(n) < pin_w) {
                 double re[n + 1], im[n + 1];
                 unpack_block_degree<n>(sv, re, im);
-                forward<n, true, PM>(R, fs[n], re, im);
+                forward<n, true>(R, fs[n], re, im);
                 keep(nc, re, im);
             } else {
                 clear(nc);
             }
+            sched_fence();
             if constexpr (n < PE) {
                 constexpr int N = n;
                 double2 cur[PO];
-                static_for<0, N + 1>([&](auto jc) { cur[decltype(jc)::value] = tg[decltype(jc)::value]; });
-                if constexpr (N > 0) fetch_degree<N - 1>(D, tg, fetch_on(N - 1));
+                take(ic<N>{}, cur);
+                prefetch(ic<N - kAhead>{});
                 sched_fence();
                 if (!RT || uint32_t(N) < pout_w) {
                     double re[N + 1], im[N + 1];
@@ -529,7 +504,7 @@ __global__ void __launch_bounds__(64)
                         re[m] = ar;
                         im[m] = ai;
                     });
-                    backward<N, true, PM>(R, bs[N], re, im);
+                    backward<N, true>(R, bs[N], re, im);
                     add_store_degree<N>(D, cur, re, im, out_on(N));
                 }
             }
