@@ -105,11 +105,15 @@ __device__ __forceinline__ void rot(const typename Pol::T *c, const typename Pol
 }
 
 template <int N, int CAP>
-__device__ __forceinline__ void fetch_degree(const double *__restrict__ blk, double2 (&v)[CAP], bool on = true)
+__device__ __forceinline__ void fetch_degree(const double *__restrict__ blk, double2 (&v)[CAP])
 {
     const double2 *q = reinterpret_cast<const double2 *>(blk + N * (N + 1));
-    static_for<0, N + 1>([&](auto jc) { v[decltype(jc)::value] = on ? q[decltype(jc)::value] : double2{0.0, 0.0}; });
+    static_for<0, N + 1>([&](auto jc) { v[decltype(jc)::value] = q[decltype(jc)::value]; });
 }
+
+// Zeros read in place of a target that is not accumulated (acc == 0): the
+// loads are never skipped, so no branch splits the load stream.
+inline __device__ double kZeroTarget[kMaxOrder * (kMaxOrder + 1)];
 
 template <int N, int CAP>
 __device__ __forceinline__ double piece(const double2 (&v)[CAP], int k)
@@ -128,6 +132,7 @@ __global__ void __launch_bounds__(64)
     const sfmm_item it = items[t];
     const double *__restrict__ S = static_cast<const double *>(__builtin_assume_aligned(src_pool + it.src, 16));
     double *__restrict__ D = static_cast<double *>(__builtin_assume_aligned(dst_pool + it.dst, 16));
+    const double *__restrict__ Dr = acc ? D : kZeroTarget;  // old target values
     const double x = it.x, y = it.y, z = it.z;
 
     // Euler data (microkernel_generic.cpp:61-96) in FP64, powers converted.
@@ -169,7 +174,7 @@ __global__ void __launch_bounds__(64)
         double2 cur[P];
         static_for<0, n + 1>([&](auto jc) { cur[decltype(jc)::value] = sv[decltype(jc)::value]; });
         if constexpr (n + 1 < P) fetch_degree<n + 1>(S, sv);
-        else fetch_degree<P - 1>(D, tg, acc != 0);
+        else fetch_degree<P - 1>(Dr, tg);
         sched_fence();
         T re[n + 1], im[n + 1];
         static_for<0, n + 1>([&](auto mc) {
@@ -201,7 +206,7 @@ __global__ void __launch_bounds__(64)
         constexpr int N = decltype(Nc)::value;
         double2 old[P];
         static_for<0, N + 1>([&](auto jc) { old[decltype(jc)::value] = tg[decltype(jc)::value]; });
-        if constexpr (N > 0) fetch_degree<N - 1>(D, tg, acc != 0);
+        if constexpr (N > 0) fetch_degree<N - 1>(Dr, tg);
         sched_fence();
         T re[N + 1], im[N + 1];
         static_for<0, N + 1>([&](auto mc) {
