@@ -86,10 +86,9 @@ __device__ __forceinline__ void swap_mfma(const double *tab, int lane, double (&
         // inputs of order l = 4c + q >= 1 doubled
         static_for<0, NC>([&](auto cc) {
             constexpr int c = decltype(cc)::value;
-            if (c > 0 || k > 0) {
-                xr[c] *= 2.0;
-                xi[c] *= 2.0;
-            }
+            const double f = (c > 0 || k > 0) ? 2.0 : 1.0;  // a select, not a branch
+            xr[c] *= f;
+            xi[c] *= f;
         });
     }
     dbl4 dr[NB], di[NB];
@@ -104,8 +103,11 @@ __device__ __forceinline__ void swap_mfma(const double *tab, int lane, double (&
             const bool in = (m <= N) && (l <= N);
             const int idx = TR ? (l * ST + m) : (m * ST + l);
             const int safe = in ? idx : 0;
-            const double ar = in ? sr[safe] : 0.0;
-            const double ai = in ? si[safe] : 0.0;
+            // always read (a valid entry), then mask by a factor: a select
+            // here would be turned into a branch around the LDS read
+            const double fin = in ? 1.0 : 0.0;
+            const double ar = sr[safe] * fin;
+            const double ai = si[safe] * fin;
             dr[b] = mfma(ar, xr[c], dr[b]);
             di[b] = mfma(ai, xi[c], di[b]);
         });
@@ -125,10 +127,9 @@ __device__ __forceinline__ void swap_mfma(const double *tab, int lane, double (&
         static_for<0, 4 * NB>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
             if constexpr (s < NSL) {
-                if (s > 0 || k > 0) {
-                    xr[s] *= 0.5;
-                    xi[s] *= 0.5;
-                }
+                const double f = (s > 0 || k > 0) ? 0.5 : 1.0;
+                xr[s] *= f;
+                xi[s] *= f;
             }
         });
     }
@@ -178,6 +179,9 @@ __device__ __forceinline__ void transform_bwd(const double *tab, int lane, const
     swap_mfma<N, TR>(tab, lane, xr, xi);
     rot<N, false, false>(R, scale, xr, xi);
 }
+
+// Read in place of a coefficient a lane does not load.
+inline __device__ double kZeros[2];
 
 // Largest value of v over the wave (0 <= v < 128), wave-uniform: one
 // ballot per bit.
@@ -253,18 +257,17 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
 
     // e^{i k theta} for k = 0..PM-1 by repeated products (as the reference
     // builds them, microkernel_generic.cpp:76-96); the lane keeps k = q+4s.
-    LaneRot<NSL> R;
+    LaneRot<NSL> R{};
     {
         double c = 1.0, s = 0.0, cb = 1.0, sb = 0.0;
         static_for<0, 4 * NSL>([&](auto kc) {
             constexpr int k = decltype(kc)::value;
             constexpr int slot = k / 4;
-            if ((k & 3) == q) {
-                R.ca[slot] = c;
-                R.sa[slot] = s;
-                R.cb[slot] = cb;
-                R.sb[slot] = sb;
-            }
+            const bool mine = (k & 3) == q;  // selects, not branches
+            R.ca[slot] = mine ? c : R.ca[slot];
+            R.sa[slot] = mine ? s : R.sa[slot];
+            R.cb[slot] = mine ? cb : R.cb[slot];
+            R.sb[slot] = mine ? sb : R.sb[slot];
             if constexpr (k + 1 < 4 * NSL) {
                 if constexpr (k == 0) {
                     c = ca1;
@@ -294,10 +297,13 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             xr[s] = 0.0;
             xi[s] = 0.0;
             if constexpr (4 * s <= n) {
-                if (active && m <= n && (!RT || uint32_t(n) < pin)) {
-                    xr[s] = S[n * (n + 1) + m];
-                    xi[s] = S[(n + 1) * (n + 1) + m];
-                }
+                // masked lanes read zeros instead of skipping the loads, so no
+                // branch splits the load stream
+                const bool ok = active && m <= n && (!RT || uint32_t(n) < pin);
+                const double *p = ok ? S + (n * (n + 1) + m) : kZeros;
+                const int im = ok ? n + 1 : 0;  // Im C_n^m follows Re C_n^m by n+1
+                xr[s] = p[0];
+                xi[s] = p[im];
             }
         });
     };
@@ -330,10 +336,11 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             oi[s] = 0.0;
             if constexpr (4 * s <= N) {
                 const int m = q + 4 * s;
-                if (active && m <= N && on && acc) {
-                    orr[s] = D[N * (N + 1) + m];
-                    oi[s] = D[(N + 1) * (N + 1) + m];
-                }
+                const bool ok = active && m <= N && on && acc;
+                const double *p = ok ? D + (N * (N + 1) + m) : kZeros;
+                const int im = ok ? N + 1 : 0;
+                orr[s] = p[0];
+                oi[s] = p[im];
             }
         });
     };
@@ -405,14 +412,10 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                             ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
                             ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
                         });
-                        if (odd) {
-                            ar = -ar;
-                            ai = -ai;
-                        }
-                        if (q + 4 * s > N) {
-                            ar = 0.0;
-                            ai = 0.0;
-                        }
+                        ar = odd ? -ar : ar;
+                        ai = odd ? -ai : ai;
+                        ar = (q + 4 * s > N) ? 0.0 : ar;
+                        ai = (q + 4 * s > N) ? 0.0 : ai;
                     }
                     xr[s] = ar;
                     xi[s] = ai;
@@ -463,10 +466,8 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                                 ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
                                 ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
                             });
-                            if (q + 4 * s > N) {
-                                ar = 0.0;
-                                ai = 0.0;
-                            }
+                            ar = (q + 4 * s > N) ? 0.0 : ar;
+                            ai = (q + 4 * s > N) ? 0.0 : ai;
                         }
                         xr[s] = ar;
                         xi[s] = ai;
@@ -527,10 +528,8 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                                 ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
                                 ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
                             });
-                            if (q + 4 * s > N) {
-                                ar = 0.0;
-                                ai = 0.0;
-                            }
+                            ar = (q + 4 * s > N) ? 0.0 : ar;
+                            ai = (q + 4 * s > N) ? 0.0 : ai;
                         }
                         xr[s] = ar;
                         xi[s] = ai;
@@ -567,8 +566,15 @@ __global__ void __launch_bounds__(64 * kWaves)
     const int lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t groups = (n_items + 15u) / 16u;
-    for (uint32_t g = blockIdx.x * kWaves + wave; g < groups; g += gridDim.x * kWaves)
-        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab, lane, acc != 0);
+    for (uint32_t g = blockIdx.x * kWaves + wave; g < groups; g += gridDim.x * kWaves) {
+        // The table is the same for every group: an opaque zero offset per
+        // group keeps the compiler from hoisting all of its reads out of the
+        // loop into registers (that would need ~300 registers at order 16);
+        // each group reads its operands from LDS where they are used.
+        int base = 0;
+        asm volatile("" : "+s"(base));
+        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab + base, lane, acc != 0);
+    }
 }
 
 using fn_t = void (*)(const double *, double *, const sfmm_item *, uint32_t, const double *, int);
