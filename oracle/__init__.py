@@ -66,8 +66,9 @@ class Oracle:
         return dst
 
     def harmonic(self, kind, P, x, y, z):
-        out = np.zeros(P * (P + 1))
-        if self.lib.orc_harmonic({"R": 0, "S": 1}[kind], P, x, y, z, _ptr(out)) != 0:
+        """R or S (P(P+1) reals), or dR (3 blocks of P(P+1): d/dx, d/dy, d/dz)."""
+        out = np.zeros((3 if kind == "dR" else 1) * P * (P + 1))
+        if self.lib.orc_harmonic({"R": 0, "S": 1, "dR": 2}[kind], P, x, y, z, _ptr(out)) != 0:
             raise ValueError("bad harmonic request")
         return out
 
@@ -141,8 +142,8 @@ class Reference:
         return v.value
 
     def harmonic(self, kind, P, x, y, z):
-        out = np.zeros(P * (P + 1))
-        self.lib.ref_harmonic({"R": 0, "S": 1}[kind], P, x, y, z, _ptr(out))
+        out = np.zeros((3 if kind == "dR" else 1) * P * (P + 1))
+        self.lib.ref_harmonic({"R": 0, "S": 1, "dR": 2}[kind], P, x, y, z, _ptr(out))
         return out
 
     def check_errors(self, op, P_op, P_in, dim_in, P_out, dim_out):

@@ -522,11 +522,72 @@ int orc_translate_naive(int op, const orc_item *items, size_t n, const double *s
 }
 
 /* kind 0: R, 1: S. */
+/* Gradient of R: three blocks of P(P+1) reals (d/dx, d/dy, d/dz), by the
+ * same recurrences as harm_R differentiated term by term (forward automatic
+ * differentiation, harmonics.cpp:81-174). */
+static void harm_dR(int P, double x, double y, double z, double *g)
+{
+    const size_t B = (size_t)P * (P + 1);
+    memset(g, 0, sizeof(double) * 3 * B);
+    if (P < 2) return;
+    double *v = (double *)calloc(B, sizeof(double));
+    if (!v) return;
+#define G(d, k) g[(size_t)(d) * B + (k)]
+    v[RE(0, 0)] = 1;
+    v[RE(1, 0)] = z;     G(2, RE(1, 0)) = 1;
+    v[RE(1, 1)] = x / 2; G(0, RE(1, 1)) = 0.5;
+    v[IM(1, 1)] = y / 2; G(1, IM(1, 1)) = 0.5;
+    for (int m = 2; m < P; ++m) {
+        const double a = v[RE(m - 1, m - 1)], b = v[IM(m - 1, m - 1)], f = 1.0 / (2.0 * m);
+        double ga[3], gb[3];
+        for (int d = 0; d < 3; ++d) {
+            ga[d] = G(d, RE(m - 1, m - 1));
+            gb[d] = G(d, IM(m - 1, m - 1));
+        }
+        v[RE(m, m)] = (a * x - b * y) * f;
+        G(0, RE(m, m)) = (ga[0] * x + a - gb[0] * y) * f;
+        G(1, RE(m, m)) = (ga[1] * x - gb[1] * y - b) * f;
+        G(2, RE(m, m)) = (ga[2] * x - gb[2] * y) * f;
+        v[IM(m, m)] = (a * y + b * x) * f;
+        G(0, IM(m, m)) = (ga[0] * y + gb[0] * x + b) * f;
+        G(1, IM(m, m)) = (ga[1] * y + a + gb[1] * x) * f;
+        G(2, IM(m, m)) = (ga[2] * y + gb[2] * x) * f;
+    }
+    for (int m = 1; m < P - 1; ++m) {
+        for (int part = 0; part < 2; ++part) {
+            const int src = part ? IM(m, m) : RE(m, m), dst = part ? IM(m + 1, m) : RE(m + 1, m);
+            v[dst] = z * v[src];
+            G(0, dst) = z * G(0, src);
+            G(1, dst) = z * G(1, src);
+            G(2, dst) = v[src] + z * G(2, src);
+        }
+    }
+    const double r2 = x * x + y * y + z * z, r2d[3] = {2 * x, 2 * y, 2 * z};
+    for (int m = 0; m < P - 2; ++m)
+        for (int n = m + 2; n < P; ++n) {
+            const double f = 1.0 / ((double)(n + m) * (double)(n - m));
+            for (int part = 0; part < 2; ++part) {
+                const int k = part ? IM(n, m) : RE(n, m), k1 = part ? IM(n - 1, m) : RE(n - 1, m),
+                          k2 = part ? IM(n - 2, m) : RE(n - 2, m);
+                const double a = v[k1], b = v[k2];
+                v[k] = (a * z * (2 * n - 1) - r2 * b) * f;
+                for (int d = 0; d < 3; ++d) {
+                    double t = G(d, k1) * z * (2 * n - 1) - r2d[d] * b - r2 * G(d, k2);
+                    if (d == 2) t += a * (2 * n - 1);
+                    G(d, k) = t * f;
+                }
+            }
+        }
+#undef G
+    free(v);
+}
+
 int orc_harmonic(int kind, int P, double x, double y, double z, double *out)
 {
     if (P < 0 || P > 4 * ORC_PMAX) return -1;
     if (kind == 0) harm_R(P, x, y, z, out);
     else if (kind == 1) harm_S(P, x, y, z, out);
+    else if (kind == 2) harm_dR(P, x, y, z, out);
     else return -1;
     return 0;
 }

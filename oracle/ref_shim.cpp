@@ -231,9 +231,11 @@ int ref_swap_entry(int n, int m, int l, double *out)
 int ref_harmonic(int kind, size_t P, double x, double y, double z, double *out)
 {
     try {
-        solidfmm::solid<double> s = (kind == 0) ? solidfmm::harmonics::R<double>(P, x, y, z)
-                                                : solidfmm::harmonics::S<double>(P, x, y, z);
-        if (P) std::memcpy(out, s.memptr(), sizeof(double) * P * (P + 1));
+        solidfmm::solid<double> s = (kind == 0)   ? solidfmm::harmonics::R<double>(P, x, y, z)
+                                    : (kind == 1) ? solidfmm::harmonics::S<double>(P, x, y, z)
+                                                  : solidfmm::harmonics::dR<double>(P, x, y, z);
+        const size_t dims = (kind == 2) ? 3 : 1;
+        if (P) std::memcpy(out, s.memptr(), sizeof(double) * dims * P * (P + 1));
         return 0;
     } catch (const std::exception &) {
         return -2;

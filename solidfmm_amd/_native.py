@@ -51,6 +51,10 @@ SIGNATURES = {
     "sfmm_plan_prepare": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_plan_bind": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
+    "sfmm_cellset_create": (_c.c_int, [_vp, _vp, _c.c_size_t, _c.c_size_t, _c.POINTER(_vp)]),
+    "sfmm_cellset_destroy": (None, [_vp]),
+    "sfmm_p2m": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
+    "sfmm_l2p": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _vp, _vp]),
     "sfmm_plan_destroy": (None, [_vp]),
     "sfmm_plan_launches": (_c.c_size_t, [_vp]),
     "sfmm_plan_extents": (None, [_vp, _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
@@ -298,6 +302,50 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None):
             load().sfmm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+CELL_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("offset", "<u8"), ("first", "<u4"),
+                       ("count", "<u4"), ("order", "<u4"), ("pad", "<u4")])
+assert CELL_DTYPE.itemsize == 48
+
+
+class CellSet:
+    """Validated device-resident cells for P2M / L2P (sfmm_cellset)."""
+
+    def __init__(self, ctx, cells, n_points):
+        cells = np.ascontiguousarray(cells, dtype=CELL_DTYPE)
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        rc = load().sfmm_cellset_create(ctx.handle, cells.ctypes.data, len(cells), n_points, ctypes.byref(h))
+        _check(rc, ctx.handle, "sfmm_cellset_create")
+        self._h = h
+
+    def p2m(self, points, pool, stream=None):
+        """pool += P2M of points ((n, 4) float64 device tensor: x, y, z, q)."""
+        pp, pn = _dev_ptr(points)
+        mp, mn = _dev_ptr(pool)
+        _check(load().sfmm_p2m(self._h, pp, pn // 4, mp, mn, _stream_ptr(stream)), self.ctx.handle, "sfmm_p2m")
+
+    def l2p(self, points, pool, phi, grad=None, stream=None):
+        """phi += potential, grad += gradient of the local expansions at the points."""
+        pp, pn = _dev_ptr(points)
+        lp, ln = _dev_ptr(pool)
+        fp, fn = _dev_ptr(phi)
+        gp, gn = _dev_ptr(grad) if grad is not None else (None, 3 * fn)
+        if fn < pn // 4 or gn < 3 * (pn // 4):
+            raise LogicError("sfmm_l2p: phi needs one real and grad three reals per point")
+        _check(load().sfmm_l2p(self._h, pp, pn // 4, lp, ln, fp, gp, _stream_ptr(stream)), self.ctx.handle, "sfmm_l2p")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().sfmm_cellset_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -726,6 +726,83 @@ int sfmm_translate_host(sfmm_context *ctx, int op, const sfmm_item *items, size_
     return rc;
 }
 
+struct sfmm_cellset {
+    sfmm_context *ctx = nullptr;
+    sfmm_cell *d_cells = nullptr;
+    size_t n_cells = 0, n_points = 0, pool_extent = 0;
+};
+
+int sfmm_cellset_create(sfmm_context *ctx, const sfmm_cell *cells, size_t n, size_t n_points, sfmm_cellset **out)
+{
+    if (!ctx || !out) return SFMM_ERR_LOGIC;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *out = nullptr;
+    if (n && !cells) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null cell list");
+    size_t extent = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const sfmm_cell &c = cells[i];
+        if (c.order > ctx->P) return fail(ctx, SFMM_ERR_RANGE, "solidfmm: cell order exceeding operator data.");
+        if (c.offset & 1u) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: expansion offsets must be even");
+        if (uint64_t(c.first) + c.count > n_points) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: cell points out of range");
+        extent = std::max<size_t>(extent, c.offset + size_t(c.order) * (c.order + 1));
+    }
+    sfmm_cellset *set = new (std::nothrow) sfmm_cellset;
+    if (!set) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
+    set->ctx = ctx;
+    set->n_cells = n;
+    set->n_points = n_points;
+    set->pool_extent = extent;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess && n) {
+        e = hipMalloc(&set->d_cells, n * sizeof(sfmm_cell));
+        if (e == hipSuccess) e = hipMemcpy(set->d_cells, cells, n * sizeof(sfmm_cell), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        if (set->d_cells) (void)hipFree(set->d_cells);
+        delete set;
+        return hip_fail(ctx, e, "cell upload");
+    }
+    *out = set;
+    return SFMM_OK;
+}
+
+void sfmm_cellset_destroy(sfmm_cellset *set)
+{
+    if (!set) return;
+    if (set->d_cells) (void)hipFree(set->d_cells);
+    delete set;
+}
+
+int sfmm_p2m(sfmm_cellset *set, const double *points, size_t n_points, double *pool, size_t pool_len, void *stream)
+{
+    if (!set) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = set->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (n_points < set->n_points || pool_len < set->pool_extent)
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: points or pool shorter than the cell set needs");
+    if (set->n_cells && (!points || !pool)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pointer");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = sfmm::launch_p2m(points, set->d_cells, set->n_cells, pool, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "p2m launch");
+    return SFMM_OK;
+}
+
+int sfmm_l2p(sfmm_cellset *set, const double *points, size_t n_points, const double *pool, size_t pool_len,
+             double *phi, double *grad, void *stream)
+{
+    if (!set) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = set->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (n_points < set->n_points || pool_len < set->pool_extent)
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: points or pool shorter than the cell set needs");
+    if (set->n_cells && (!points || !pool || !phi)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pointer");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess)
+        e = sfmm::launch_l2p(points, set->d_cells, set->n_cells, pool, phi, grad, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "l2p launch");
+    return SFMM_OK;
+}
+
 int sfmm_synchronize(sfmm_context *ctx, void *stream)
 {
     if (!ctx) return SFMM_ERR_LOGIC;

@@ -100,6 +100,11 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
 hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
                           hipStream_t stream, int acc = 1);
 
+// Particle-side operators over device-resident cells (particles.hip).
+hipError_t launch_p2m(const double *pts, const sfmm_cell *cells, size_t n_cells, double *pool, hipStream_t stream);
+hipError_t launch_l2p(const double *pts, const sfmm_cell *cells, size_t n_cells, const double *pool, double *phi,
+                      double *grad, hipStream_t stream);
+
 // Enqueue dst += T(src) for n items all of orders (p_in, p_out) with
 // pairwise distinct targets, in `family` (kAuto: default_family).
 hipError_t launch_translate(int op, uint32_t p_in, uint32_t p_out, const sfmm_item *items, size_t n,

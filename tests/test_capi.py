@@ -26,7 +26,7 @@ def lib():
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sfmm_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(sfmm_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_entry_points():
@@ -48,6 +48,18 @@ def test_item_layout_matches_header():
     assert ctypes.sizeof(Item) == 48 == ITEM_DTYPE.itemsize
     for name, (dt, off) in ITEM_DTYPE.fields.items():
         assert getattr(Item, name).offset == off
+
+
+def test_cell_layout_matches_header():
+    from solidfmm_amd import CELL_DTYPE
+
+    class Cell(ctypes.Structure):
+        _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double),
+                    ("offset", ctypes.c_uint64), ("first", ctypes.c_uint32), ("count", ctypes.c_uint32),
+                    ("order", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+    assert ctypes.sizeof(Cell) == 48 == CELL_DTYPE.itemsize
+    for name, (dt, off) in CELL_DTYPE.fields.items():
+        assert getattr(Cell, name).offset == off
 
 
 def test_library_reports_order_limit(lib):

@@ -13,6 +13,8 @@ Each oracle function is checked against something it does not restate:
                     exactness; M2M and L2L composition
   harmonics R, S    the compiled reference's harmonics; |x|^-1 and 1 at
                     degree 0; z-axis closed forms j!/r^(j+1) and r^j/j!
+  gradient dR       central finite differences of R; the compiled
+                    reference's harmonics::dR; the closed form on the z axis
 """
 import math
 
@@ -119,6 +121,34 @@ def test_harmonics_equal_reference(orc, ref):
             a = orc.harmonic(kind, 15, x, y, z)
             b = ref.harmonic(kind, 15, x, y, z)
             assert np.abs(a - b).max() <= 1e-15 * np.abs(b).max()
+
+
+def test_harmonic_gradient_finite_differences(orc):
+    P = 12
+    x, y, z = 0.31, -0.72, 0.55
+    g = orc.harmonic("dR", P, x, y, z).reshape(3, -1)
+    h = 1e-5
+    for d in range(3):
+        e = np.zeros(3)
+        e[d] = h
+        fd = (orc.harmonic("R", P, x + e[0], y + e[1], z + e[2])
+              - orc.harmonic("R", P, x - e[0], y - e[1], z - e[2])) / (2 * h)
+        assert np.abs(g[d] - fd).max() <= 1e-9 * max(1.0, np.abs(fd).max())
+    # on the z axis only the m = 0 (d/dz: r^(j-1)/(j-1)!) and m = 1 (d/dx, d/dy)
+    # entries survive
+    r = 1.3
+    g = orc.harmonic("dR", P, 0.0, 0.0, r).reshape(3, -1)
+    for j in range(1, P):
+        assert g[2][j * (j + 1)] == pytest.approx(r ** (j - 1) / math.factorial(j - 1), rel=1e-14)
+
+
+@pytest.mark.reference
+def test_harmonic_gradient_equals_reference(orc, ref):
+    for P in (1, 2, 5, 12):
+        for (x, y, z) in [(0.3, -1.1, 0.8), (1.0, 2.0, -0.5)]:
+            a = orc.harmonic("dR", P, x, y, z)
+            b = ref.harmonic("dR", P, x, y, z)
+            assert np.abs(a - b).max() <= 1e-15 * max(1.0, np.abs(b).max())
 
 
 def _potential_after(orc, op, P):
