@@ -234,6 +234,22 @@ int sfmm_p2m(sfmm_cellset *set, const double *dev_points, size_t n_points, doubl
 int sfmm_l2p(sfmm_cellset *set, const double *dev_points, size_t n_points, const double *dev_pool, size_t pool_len,
              double *dev_phi, double *dev_grad, void *stream);
 
+/* Smallest enclosing ball (host code, no context): of n points given as
+ * coordinate arrays, or of n balls given as (x, y, z, r) quadruples;
+ * out = (x, y, z, r).  n == 0 or a negative radius is SFMM_ERR_LOGIC.
+ * Replaces solidfmm::bounding_sphere (sphere.hpp:38-42, sphere.cpp:546-627). */
+int sfmm_bounding_sphere(size_t n, const double *x, const double *y, const double *z, double *out);
+int sfmm_bounding_sphere_of_spheres(size_t n, const double *spheres, double *out);
+
+/* Batched form for trees: the smallest ball of points[first[s] ..
+ * first[s] + count[s]) for every set s, points being rows of `stride`
+ * doubles whose first three are x, y, z; out holds n_sets quadruples.
+ * Runs on `threads` host threads (0: all cores).  Empty sets and ranges
+ * beyond n_points are SFMM_ERR_LOGIC.  No counterpart in the reference
+ * (its users loop over the single-set form). */
+int sfmm_bounding_spheres_of_ranges(const double *points, size_t n_points, size_t stride, const uint64_t *first,
+                                    const uint64_t *count, size_t n_sets, double *out, int threads);
+
 /* Blocks until all work the context issued on `stream` is done. */
 int sfmm_synchronize(sfmm_context *ctx, void *stream);
 

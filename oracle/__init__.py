@@ -108,6 +108,7 @@ class Reference:
         L.ref_swap_entry.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]
         L.ref_harmonic.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_double, _dp]
+        L.ref_bounding_sphere.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp]
         L.ref_check_errors.argtypes = [ctypes.c_int] + [ctypes.c_size_t] * 5
 
     @staticmethod
@@ -144,6 +145,21 @@ class Reference:
     def harmonic(self, kind, P, x, y, z):
         out = np.zeros((3 if kind == "dR" else 1) * P * (P + 1))
         self.lib.ref_harmonic({"R": 0, "S": 1, "dR": 2}[kind], P, x, y, z, _ptr(out))
+        return out
+
+    def bounding_sphere(self, points=None, spheres=None):
+        """The reference's smallest enclosing ball (x, y, z, r) of points
+        (n, 3) or of balls (n, 4)."""
+        out = np.zeros(4)
+        if spheres is not None:
+            data = np.ascontiguousarray(spheres, dtype=np.float64).reshape(-1)
+            kind, n = 1, len(data) // 4
+        else:
+            p = np.asarray(points, dtype=np.float64)
+            data = np.ascontiguousarray(p[:, :3].T).reshape(-1)
+            kind, n = 0, len(p)
+        if self.lib.ref_bounding_sphere(kind, n, _ptr(data), _ptr(out)) != 0:
+            raise ValueError("reference bounding_sphere raised")
         return out
 
     def check_errors(self, op, P_op, P_in, dim_in, P_out, dim_out):

@@ -23,6 +23,7 @@
 // point on N std::threads instead, each owning the targets of one hash range
 // (the partitioning multithreaded_translator.cpp:298-319 uses), with its own
 // buffer_handle.
+#include <solidfmm/sphere.hpp>
 #include <solidfmm/handles.hpp>
 #include <solidfmm/harmonics.hpp>
 #include <solidfmm/solid.hpp>
@@ -264,6 +265,30 @@ int ref_check_errors(int op, size_t P_op, size_t P_in, size_t dim_in, size_t P_o
         return 1;
     } catch (...) {
         return 4;
+    }
+}
+
+// Smallest enclosing ball through the reference (sphere.cpp:546-627):
+// kind 0 of n points (x, y, z arrays in data[0..3n)), kind 1 of n balls
+// (data = n quadruples x, y, z, r).  out = x, y, z, r.  1 on a C++ exception.
+int ref_bounding_sphere(int kind, size_t n, const double *data, double *out)
+{
+    try {
+        solidfmm::sphere<double> s;
+        if (kind == 0) {
+            s = solidfmm::bounding_sphere(n, data, data + n, data + 2 * n);
+        } else {
+            std::vector<solidfmm::sphere<double>> b(n);
+            for (size_t i = 0; i < n; ++i) b[i] = {data[4 * i], data[4 * i + 1], data[4 * i + 2], data[4 * i + 3]};
+            s = solidfmm::bounding_sphere(b.data(), b.data() + n);
+        }
+        out[0] = s.x;
+        out[1] = s.y;
+        out[2] = s.z;
+        out[3] = s.r;
+        return 0;
+    } catch (const std::exception &) {
+        return 1;
     }
 }
 

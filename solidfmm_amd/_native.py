@@ -51,6 +51,10 @@ SIGNATURES = {
     "sfmm_plan_prepare": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_plan_bind": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
+    "sfmm_bounding_sphere": (_c.c_int, [_c.c_size_t, _vp, _vp, _vp, _vp]),
+    "sfmm_bounding_sphere_of_spheres": (_c.c_int, [_c.c_size_t, _vp, _vp]),
+    "sfmm_bounding_spheres_of_ranges": (_c.c_int, [_vp, _c.c_size_t, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp,
+                                                   _c.c_int]),
     "sfmm_cellset_create": (_c.c_int, [_vp, _vp, _c.c_size_t, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_cellset_destroy": (None, [_vp]),
     "sfmm_p2m": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
@@ -353,3 +357,33 @@ class CellSet:
             self.close()
         except Exception:
             pass
+
+
+def bounding_sphere(points=None, spheres=None):
+    """Smallest enclosing ball (x, y, z, r) of an (n, >=3) array of points or
+    of an (n, 4) array of balls (solidfmm::bounding_sphere)."""
+    out = np.zeros(4)
+    if spheres is not None:
+        b = np.ascontiguousarray(spheres, dtype=np.float64).reshape(-1, 4)
+        _check(load().sfmm_bounding_sphere_of_spheres(len(b), b.ctypes.data, out.ctypes.data), None,
+               "sfmm_bounding_sphere_of_spheres")
+        return out
+    p = np.asarray(points, dtype=np.float64)
+    x, y, z = (np.ascontiguousarray(p[:, d]) for d in range(3))
+    _check(load().sfmm_bounding_sphere(len(p), x.ctypes.data, y.ctypes.data, z.ctypes.data, out.ctypes.data), None,
+           "sfmm_bounding_sphere")
+    return out
+
+
+def bounding_spheres(points, first, count, threads=0):
+    """Smallest balls of points[first[s]:first[s]+count[s]] for every s, as an
+    (n_sets, 4) array; points is an (n, >=3) float64 array (rows of any
+    width, the first three columns are x, y, z)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    f = np.ascontiguousarray(first, dtype=np.uint64)
+    c = np.ascontiguousarray(count, dtype=np.uint64)
+    out = np.zeros((len(f), 4))
+    _check(load().sfmm_bounding_spheres_of_ranges(p.ctypes.data, p.shape[0], p.shape[1], f.ctypes.data, c.ctypes.data,
+                                                  len(f), out.ctypes.data, threads), None,
+           "sfmm_bounding_spheres_of_ranges")
+    return out

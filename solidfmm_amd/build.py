@@ -27,10 +27,11 @@ COMMON = [
 # Translation units; the kernel instantiations are split so they compile in
 # parallel.
 SOURCES = ["lane_prec_dw.hip", "mfma_m2l_rt.hip", "mfma_m2l.hip", "mfma_m2m_rt.hip", "mfma_l2l_rt.hip", "mfma_m2m.hip",
-           "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "reduce.hip", "particles.hip", "translate.hip", "generic.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp"]
+           "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "reduce.hip", "particles.hip", "translate.hip", "generic.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp",
+           "sphere.cpp"]
 
 # Host-only C++ (the drop-in API over the C ABI).
-HOST_ONLY = {"dropin.cpp"}
+HOST_ONLY = {"dropin.cpp", "sphere.cpp"}
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 
@@ -56,6 +57,7 @@ def _stale(src):
     t = os.path.getmtime(obj)
     deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
     deps += [os.path.join(INCLUDE, "solidfmm_gpu.h")]
+    deps += [os.path.join(INCLUDE, "solidfmm", h) for h in os.listdir(os.path.join(INCLUDE, "solidfmm"))]
     return any(os.path.getmtime(d) > t for d in deps)
 
 
