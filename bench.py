@@ -301,7 +301,11 @@ def serving_mode(args):
     seed = 0x5011DF04
     t0 = time.perf_counter()
     tree = O.Octree(O.plummer(args.points, seed), leaf_size=64)
-    P, off, pool_len, passes = O.translation_lists(tree, 20, 6)
+    if args.lists == "mac":
+        from solidfmm_amd import fmm as F
+        P, off, pool_len, passes = F.mac_translation_lists(tree, O.plummer(args.points, seed), args.theta, 20, 6)
+    else:
+        P, off, pool_len, passes = O.translation_lists(tree, 20, 6)
     m0 = W.uniform(seed, 7, pool_len, -1.0, 1.0)
     build_s = time.perf_counter() - t0
     results = []
@@ -333,12 +337,150 @@ def serving_mode(args):
             "data": "synthetic",
             "config": {"workload": f"Plummer sphere of {args.points} points (a=1, seeded), octree <= 64 per leaf "
                                    f"({len(tree)} cells, depth {tree.depth}), orders 20 (root) to 6 (leaves); "
-                                   "M2M/M2L/L2L passes cut into requests of U[1k,50k] translations, Poisson "
+                                   + (f"M2L lists by dual tree walk, theta={args.theta}, smallest-ball centres; "
+                                      if args.lists == "mac" else "M2L by same-level V-lists; ")
+                                   + "M2M/M2L/L2L passes cut into requests of U[1k,50k] translations, Poisson "
                                    f"arrivals at {args.load} x one-at-a-time capacity; {args.slots} streams, "
                                    f"{args.budget_mib} MiB device arena for requests in flight",
                        "points": args.points, "cells": int(len(tree)), "depth": int(tree.depth),
                        "tree_and_lists_s": build_s},
             "latency_ms": head["latency_ms"], "results": results}
+    print(json.dumps(line), flush=True)
+
+
+def fmm_mode(args):
+    """Whole far-field FMM evaluation of a Plummer sphere (--points, leaf
+    size 64, acceptance parameter --theta, orders 20 at the root to 6 at the
+    leaves): P2M, M2M, M2L, L2L and L2P on the GPU, timed over --steps
+    evaluations; the error of potential and gradient (far field plus the
+    host near field) against direct sums on 64 sampled points."""
+    import torch
+
+    import solidfmm_amd
+    from solidfmm_amd import octree as O
+    from solidfmm_amd.fmm import FMM
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    solidfmm_amd.load()
+    seed = 0x5011DF04
+    rng = np.random.default_rng(seed)
+    pts = np.c_[O.plummer(args.points, seed), rng.uniform(0.5, 1.5, args.points) / args.points]
+    t0 = time.perf_counter()
+    ctx = solidfmm_amd.Context(20)
+    fmm = FMM(ctx, pts, leaf_size=64, theta=args.theta, p_root=20, p_leaf=6)
+    setup_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        fmm.evaluate()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        fmm.evaluate()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    marks = []
+
+    def mark(name):
+        ev = torch.cuda.Event(enable_timing=True)
+        ev.record()
+        marks.append((name, ev))
+
+    phi, grad = fmm.evaluate(mark=mark)
+    torch.cuda.synchronize()
+    phases = {}
+    for (_, a), (name, b) in zip(marks[:-1], marks[1:]):
+        phases[name] = phases.get(name, 0.0) + a.elapsed_time(b)
+    phi = phi.cpu().numpy()
+    grad = grad.cpu().numpy()
+    ep, eg = [], []
+    for i in rng.choice(args.points, 64, replace=False):
+        d = pts[i, :3] - pts[:, :3]
+        r = np.linalg.norm(d, axis=1)
+        r[i] = np.inf
+        want_p = (pts[:, 3] / r).sum()
+        want_g = -(pts[:, 3, None] * d / r[:, None] ** 3).sum(0)
+        near_p, near_g = fmm.near_field_at(pts, i)
+        ep.append(abs(phi[i] + near_p - want_p) / abs(want_p))
+        eg.append(np.linalg.norm(grad[i] + near_g - want_g) / np.linalg.norm(want_g))
+    n_tr = fmm.translations
+    line = {"metric": "far-field FMM evaluations of a Plummer sphere: translations/s over P2M..L2P",
+            "value": n_tr / (ms * 1e-3), "unit": "translations/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Plummer sphere of {args.points} points (a=1, seeded, charges U[0.5,1.5]/n), "
+                                   f"octree <= 64 per leaf ({len(fmm.tree)} cells, depth {fmm.tree.depth}), "
+                                   f"smallest-ball centres, dual tree walk theta={args.theta}, orders 20 (root) to "
+                                   "6 (leaves); one step = P2M + M2M + M2L + L2L + L2P (near field not included)",
+                       "points": args.points, "cells": int(len(fmm.tree)), "translations": n_tr,
+                       "m2l": int(len(fmm.m2l_src)), "near_pairs": int(len(fmm.near_src)), "setup_s": setup_s},
+            "points_per_s": args.points / (ms * 1e-3), "phase_ms": phases,
+            "rel_err_vs_direct": {"potential_max": float(np.max(ep)), "potential_p50": float(np.median(ep)),
+                                  "gradient_max": float(np.max(eg)), "gradient_p50": float(np.median(eg))}}
+    print(json.dumps(line), flush=True)
+
+
+def host_mode(args):
+    """The host-memory path (PCIe-inclusive): expansions live in host pools
+    and every call copies the distinct source and target blocks in, runs the
+    plan and copies the targets back (sfmm_staging_* / sfmm_translate_host,
+    the path of the C++ drop-in).  Two lists: the headline batch (--n
+    independent M2L at --P, every block moved once per translation) and the
+    M2L list of a Plummer FMM (--points, theta --theta, orders 20..6), where
+    sources are shared by many targets.  Reports translations/s of
+    (a) pinned staging pools already packed, (b) pageable host pools
+    (translate_host, which packs them with host threads)."""
+    import torch
+
+    import solidfmm_amd
+    from solidfmm_amd import fmm as F
+    from solidfmm_amd import octree as O
+    from solidfmm_amd import workload as W
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    solidfmm_amd.load()
+    rows = []
+    items, src, dst = W.uniform_batch(W.M2L, args.P, args.P, args.n, 0x5011DF00)
+    lists = [("independent M2L P=%d, %d items" % (args.P, args.n), args.P, items, src, dst)]
+    tree = O.Octree(O.plummer(args.points, 0x5011DF04), leaf_size=64)
+    P, off, pool_len, passes = F.mac_translation_lists(tree, O.plummer(args.points, 0x5011DF04), args.theta, 20, 6)
+    m2l = [p for p in passes if p[0] == W.M2L][0][1]
+    m0 = W.uniform(0x5011DF04, 7, pool_len, -1.0, 1.0)
+    lists.append(("Plummer FMM M2L list, %d points, theta %.2f, orders 20..6" % (args.points, args.theta), 20, m2l,
+                  m0, np.zeros(pool_len)))
+    for name, Pc, it, s_pool, d_pool in lists:
+        ctx = solidfmm_amd.Context(Pc)
+        st = solidfmm_amd.Staging(ctx)
+        hs, hd = st.reserve(len(s_pool), len(d_pool))
+        hs[:] = s_pool
+        hd[:] = d_pool
+        st.translate(W.M2L, it, len(s_pool), len(d_pool))  # warm: buffers grown
+        t = []
+        for _ in range(max(args.steps // 50, 3)):
+            t0 = time.perf_counter()
+            st.translate(W.M2L, it, len(s_pool), len(d_pool))
+            t.append(time.perf_counter() - t0)
+        pinned_s = float(np.median(t))
+        dpool = d_pool.copy()
+        ctx.translate_host(W.M2L, it, s_pool, dpool)
+        t = []
+        for _ in range(max(args.steps // 50, 3)):
+            t0 = time.perf_counter()
+            ctx.translate_host(W.M2L, it, s_pool, dpool)
+            t.append(time.perf_counter() - t0)
+        pageable_s = float(np.median(t))
+        moved = 8 * (len(s_pool) + 2 * len(d_pool))
+        rows.append({"list": name, "items": int(len(it)), "pcie_bytes": moved,
+                     "pinned": {"s": pinned_s, "translations_per_s": len(it) / pinned_s,
+                                "GB_per_s": moved / pinned_s / 1e9},
+                     "pageable": {"s": pageable_s, "translations_per_s": len(it) / pageable_s}})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        del st, ctx
+    line = {"metric": "host-memory path: translations/s including PCIe copies (sfmm_staging / translate_host)",
+            "value": rows[0]["pageable"]["translations_per_s"], "unit": "translations/s", "n_gpus": 1,
+            "steps": max(args.steps // 50, 3), "warmup": 1, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "host pools: " + "; ".join(r["list"] for r in rows)}, "results": rows}
     print(json.dumps(line), flush=True)
 
 
@@ -354,7 +496,7 @@ def main():
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "serving", "copy"], default="translate",
+    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "serving", "fmm", "host", "copy"], default="translate",
                     help="mixed: the mixed-order M2L workload (orders --pmin..--pmax, default 500,000 items) "
                          "as one captured HIP graph; copy: time a plain device copy of --copy-mib MiB (HBM "
                          "reference and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -370,6 +512,9 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--theta", type=float, default=0.6, help="fmm/serving: acceptance parameter of the MAC lists")
+    ap.add_argument("--lists", choices=["mac", "vlist"], default="mac",
+                    help="serving: dual-tree MAC lists or same-level V-lists")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -382,6 +527,10 @@ def main():
         return mixed_mode(args)
     if args.mode == "precision":
         return precision_mode(args)
+    if args.mode == "fmm":
+        return fmm_mode(args)
+    if args.mode == "host":
+        return host_mode(args)
     if args.mode == "serving":
         return serving_mode(args)
 

@@ -234,6 +234,26 @@ int sfmm_p2m(sfmm_cellset *set, const double *dev_points, size_t n_points, doubl
 int sfmm_l2p(sfmm_cellset *set, const double *dev_points, size_t n_points, const double *dev_pool, size_t pool_len,
              double *dev_phi, double *dev_grad, void *stream);
 
+/* Host-pool staging for callers whose expansions live in host memory (the
+ * drop-in's solid* lists, FFI callers with host arrays).  A staging object
+ * owns pinned host pools, device pools, plan memory and a stream, all
+ * grow-only and reused across calls, so a call allocates nothing once
+ * warm.  Use: sfmm_staging_reserve returns the pinned source and target
+ * pools (contents undefined); the caller packs its blocks into them (the
+ * solid.hpp layout, offsets as in sfmm_item); sfmm_staging_translate then
+ * runs dst += T(src) over them -- copies in, plan, kernels, copy out -- and
+ * returns with the result in the pinned target pool.  The host-side plan
+ * preparation overlaps the copies in.  One staging object per calling
+ * thread; objects on one context may run concurrently (disjoint pools).
+ * Replaces the per-thread workspace of the reference
+ * (threadlocal_buffer.hpp:30-70) for the host-pointer path. */
+typedef struct sfmm_staging sfmm_staging;
+int sfmm_staging_create(sfmm_context *ctx, sfmm_staging **st);
+void sfmm_staging_destroy(sfmm_staging *st);
+int sfmm_staging_reserve(sfmm_staging *st, size_t src_len, size_t dst_len, double **h_src, double **h_dst);
+int sfmm_staging_translate(sfmm_staging *st, int op, const sfmm_item *items, size_t n, size_t src_len,
+                           size_t dst_len);
+
 /* Smallest enclosing ball (host code, no context): of n points given as
  * coordinate arrays, or of n balls given as (x, y, z, r) quadruples;
  * out = (x, y, z, r).  n == 0 or a negative radius is SFMM_ERR_LOGIC.

@@ -10,7 +10,7 @@ mirrors the reference's multithreaded_translator batch interface
 from . import workload
 from .workload import ITEM_DTYPE, M2M, M2L, L2L, solid_size
 from ._native import (SFMM_FP64, SFMM_FP32, SFMM_SPLIT32, PRECISIONS)
-from ._native import (Context, Plan, CellSet, CELL_DTYPE, bounding_sphere, bounding_spheres, load, LIB_PATH, SolidfmmError, LogicError, OutOfRangeError,
+from ._native import (Context, Plan, CellSet, CELL_DTYPE, Staging, bounding_sphere, bounding_spheres, load, LIB_PATH, SolidfmmError, LogicError, OutOfRangeError,
                       OrderOverflowError, DeviceAllocError, HipError, UnsupportedError)
 
 
@@ -43,5 +43,5 @@ class Translator:
 
 
 __all__ = ["SFMM_FP64", "SFMM_FP32", "SFMM_SPLIT32", "PRECISIONS", "workload", "ITEM_DTYPE", "M2M", "M2L", "L2L",
-           "solid_size", "Context", "Plan", "CellSet", "CELL_DTYPE", "bounding_sphere", "bounding_spheres", "Translator", "load", "LIB_PATH", "SolidfmmError", "LogicError", "OutOfRangeError", "OrderOverflowError",
+           "solid_size", "Context", "Plan", "CellSet", "CELL_DTYPE", "Staging", "bounding_sphere", "bounding_spheres", "Translator", "load", "LIB_PATH", "SolidfmmError", "LogicError", "OutOfRangeError", "OrderOverflowError",
            "DeviceAllocError", "HipError", "UnsupportedError"]

@@ -51,6 +51,10 @@ SIGNATURES = {
     "sfmm_plan_prepare": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_plan_bind": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
+    "sfmm_staging_create": (_c.c_int, [_vp, _c.POINTER(_vp)]),
+    "sfmm_staging_destroy": (None, [_vp]),
+    "sfmm_staging_reserve": (_c.c_int, [_vp, _c.c_size_t, _c.c_size_t, _c.POINTER(_vp), _c.POINTER(_vp)]),
+    "sfmm_staging_translate": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.c_size_t, _c.c_size_t]),
     "sfmm_bounding_sphere": (_c.c_int, [_c.c_size_t, _vp, _vp, _vp, _vp]),
     "sfmm_bounding_sphere_of_spheres": (_c.c_int, [_c.c_size_t, _vp, _vp]),
     "sfmm_bounding_spheres_of_ranges": (_c.c_int, [_vp, _c.c_size_t, _c.c_size_t, _vp, _vp, _c.c_size_t, _vp,
@@ -387,3 +391,40 @@ def bounding_spheres(points, first, count, threads=0):
                                                   len(f), out.ctypes.data, threads), None,
            "sfmm_bounding_spheres_of_ranges")
     return out
+
+
+class Staging:
+    """Pinned host pools + device pools + stream for host-memory callers
+    (sfmm_staging).  reserve() returns numpy views of the pinned source and
+    target pools; translate() runs dst += T(src) over them and returns with
+    the result in the target view."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        _check(load().sfmm_staging_create(ctx.handle, ctypes.byref(h)), ctx.handle, "sfmm_staging_create")
+        self._h = h
+
+    def reserve(self, src_len, dst_len):
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(load().sfmm_staging_reserve(self._h, src_len, dst_len, ctypes.byref(a), ctypes.byref(b)),
+               self.ctx.handle, "sfmm_staging_reserve")
+        src = np.ctypeslib.as_array(ctypes.cast(a, ctypes.POINTER(ctypes.c_double)), shape=(max(src_len, 1),))
+        dst = np.ctypeslib.as_array(ctypes.cast(b, ctypes.POINTER(ctypes.c_double)), shape=(max(dst_len, 1),))
+        return src[:src_len], dst[:dst_len]
+
+    def translate(self, op, items, src_len, dst_len):
+        items = np.ascontiguousarray(items, dtype=ITEM_DTYPE)
+        _check(load().sfmm_staging_translate(self._h, op, items.ctypes.data, len(items), src_len, dst_len),
+               self.ctx.handle, "sfmm_staging_translate")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().sfmm_staging_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -24,6 +24,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -43,6 +44,9 @@ struct sfmm_context {
     int tune[3][kOrders] = {};  // autotuned family per (op, pb); 0 = default
     std::string err;
     std::mutex mu;
+    // staging of the blocking host-pool call (sfmm_translate_host)
+    std::mutex host_mu;
+    sfmm_staging *host_staging = nullptr;
 };
 
 struct sfmm_plan {
@@ -132,6 +136,56 @@ uint32_t written_reals(int op, const sfmm_item &it)
     return it.p_out * (it.p_out + 1);
 }
 
+// Open-addressing map from 64-bit keys (never ~0) to 32-bit values that
+// grows with its contents; plan building looks up every item's target in
+// it, and the distinct targets of a list are usually far fewer than its
+// items, so the table stays cache-resident.
+class FlatMap {
+public:
+    // value of k, inserting `v` first if k is new
+    uint32_t &get(uint64_t k, uint32_t v, bool &inserted)
+    {
+        if (2 * (size_ + 1) > keys_.size()) grow();
+        size_t i = slot(k);
+        while (keys_[i] != kEmpty && keys_[i] != k) i = (i + 1) & mask_;
+        inserted = keys_[i] == kEmpty;
+        if (inserted) {
+            keys_[i] = k;
+            vals_[i] = v;
+            ++size_;
+        }
+        return vals_[i];
+    }
+    size_t size() const { return size_; }
+
+private:
+    static constexpr uint64_t kEmpty = ~uint64_t(0);
+    size_t slot(uint64_t k) const { return size_t((k * 0x9E3779B97F4A7C15ull) >> shift_) & mask_; }
+    void grow()
+    {
+        const size_t cap = keys_.empty() ? 1024 : keys_.size() * 2;
+        std::vector<uint64_t> ok(cap, kEmpty);
+        std::vector<uint32_t> ov(cap);
+        ok.swap(keys_);
+        ov.swap(vals_);
+        mask_ = cap - 1;
+        shift_ = 64 - __builtin_ctzll(cap);
+        size_ = 0;
+        for (size_t i = 0; i < ok.size(); ++i)
+            if (ok[i] != kEmpty) {
+                size_t j = slot(ok[i]);
+                while (keys_[j] != kEmpty) j = (j + 1) & mask_;
+                keys_[j] = ok[i];
+                vals_[j] = ov[i];
+                ++size_;
+            }
+    }
+    std::vector<uint64_t> keys_;
+    std::vector<uint32_t> vals_;
+    size_t mask_ = 0, size_ = 0;
+    int shift_ = 64;
+};
+
 // Validation and grouping into `plan` (host side only).  Items with an
 // order of zero contribute nothing (the reference pads such lanes with
 // zeros and writes them to its trash buffer, microkernel_generic.cpp:
@@ -151,8 +205,7 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
     };
     std::vector<Key> keys;
     keys.reserve(n);
-    std::unordered_map<uint64_t, uint32_t> seen;  // target -> occurrences so far
-    seen.reserve(n * 2 + 1);
+    FlatMap seen;  // target -> occurrences so far
     uint32_t rounds = 0;
     for (size_t i = 0; i < n; ++i) {
         const sfmm_item &it = items[i];
@@ -168,7 +221,8 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
                         std::string("solidfmm: no kernel for ") + op_name(op) + " with p_in=" +
                             std::to_string(it.p_in) + ", p_out=" + std::to_string(it.p_out));
         const uint32_t pb = std::max(it.p_in, it.p_out);
-        const uint32_t round = seen[it.dst]++;
+        bool fresh;
+        const uint32_t round = seen.get(it.dst, 0, fresh)++;
         rounds = std::max(rounds, round + 1);
         plan.src_extent = std::max<size_t>(plan.src_extent, it.src + size_t(it.p_in) * (it.p_in + 1));
         plan.dst_extent = std::max<size_t>(plan.dst_extent, it.dst + size_t(it.p_out) * (it.p_out + 1));
@@ -184,13 +238,35 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         list_pos.resize(n);
         for (size_t k = 0; k < keys.size(); ++k) list_pos[keys[k].index] = uint32_t(k);
     }
-    std::stable_sort(keys.begin(), keys.end(), [](const Key &a, const Key &b) {
-        if (a.round != b.round) return a.round < b.round;
-        if (a.family != b.family) return a.family < b.family;
-        if (a.pb != b.pb) return a.pb < b.pb;
-        if (a.p_in != b.p_in) return a.p_in < b.p_in;
-        return a.p_out < b.p_out;
-    });
+    // Stable order by (round, family, pb, p_in, p_out): the packed key of
+    // each item, its rank among the (few) distinct keys, then a counting
+    // sort by rank.
+    {
+        auto pack = [](const Key &k) {
+            return (uint64_t(k.round) << 40) | (uint64_t(k.family) << 32) | (uint64_t(k.pb) << 16) |
+                   (uint64_t(k.p_in) << 8) | uint64_t(k.p_out);
+        };
+        FlatMap bucket;
+        std::vector<uint64_t> distinct;
+        std::vector<uint32_t> of(keys.size());
+        for (size_t k = 0; k < keys.size(); ++k) {
+            bool fresh;
+            const uint64_t ck = pack(keys[k]);
+            of[k] = bucket.get(ck, uint32_t(distinct.size()), fresh);
+            if (fresh) distinct.push_back(ck);
+        }
+        std::vector<uint32_t> order(distinct.size());
+        for (uint32_t b = 0; b < order.size(); ++b) order[b] = b;
+        std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return distinct[a] < distinct[b]; });
+        std::vector<size_t> start(distinct.size() + 1, 0);
+        std::vector<uint32_t> rank(distinct.size());
+        for (uint32_t r = 0; r < order.size(); ++r) rank[order[r]] = r;
+        for (size_t k = 0; k < keys.size(); ++k) start[rank[of[k]] + 1]++;
+        for (size_t r = 0; r < distinct.size(); ++r) start[r + 1] += start[r];
+        std::vector<Key> sorted(keys.size());
+        for (size_t k = 0; k < keys.size(); ++k) sorted[start[rank[of[k]]]++] = keys[k];
+        keys.swap(sorted);
+    }
     const size_t m = keys.size();
     plan.n_items = m;
     plan.launches.clear();
@@ -224,20 +300,19 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
     std::vector<sfmm::RedTarget> rt;
     std::vector<sfmm::RedItem> ri;
     if (plan.scratch) {
-        std::unordered_map<uint64_t, uint32_t> tid;
-        tid.reserve(seen.size() * 2 + 1);
+        FlatMap tid;
         std::vector<uint32_t> count;
         std::vector<uint32_t> of_pos(m);  // target id per list position
         size_t pos = 0;
         for (size_t i = 0; i < n; ++i) {
             const sfmm_item &it = items[i];
             if (it.p_in == 0 || it.p_out == 0) continue;
-            auto ins = tid.emplace(it.dst, uint32_t(rt.size()));
-            if (ins.second) {
+            bool fresh;
+            const uint32_t t = tid.get(it.dst, uint32_t(rt.size()), fresh);
+            if (fresh) {
                 rt.push_back({it.dst, 0, 0, 0, 0});
                 count.push_back(0);
             }
-            const uint32_t t = ins.first->second;
             of_pos[pos++] = t;
             count[t]++;
             rt[t].len = std::max(rt[t].len, written_reals(op, it));
@@ -325,7 +400,11 @@ int sfmm_create(sfmm_context **out, int device, size_t P, int precision)
     return SFMM_OK;
 }
 
-void sfmm_destroy(sfmm_context *ctx) { delete ctx; }
+void sfmm_destroy(sfmm_context *ctx)
+{
+    if (ctx && ctx->host_staging) sfmm_staging_destroy(ctx->host_staging);
+    delete ctx;
+}
 
 size_t sfmm_order(const sfmm_context *ctx) { return ctx ? ctx->P : 0; }
 
@@ -683,46 +762,209 @@ int sfmm_set_tuning(sfmm_context *ctx, int op, uint32_t order, int family)
     return SFMM_OK;
 }
 
+struct sfmm_staging {
+    sfmm_context *ctx = nullptr;
+    double *h_src = nullptr, *h_dst = nullptr;  // pinned
+    size_t h_src_cap = 0, h_dst_cap = 0;        // reals
+    double *d_src = nullptr, *d_dst = nullptr;
+    size_t d_src_cap = 0, d_dst_cap = 0;  // reals
+    void *d_plan = nullptr;
+    size_t d_plan_cap = 0;  // bytes
+    hipStream_t stream = nullptr;
+};
+
+}  // extern "C"
+
+namespace {
+
+// grow-only pinned host buffer of at least `want` reals
+hipError_t grow_host(double *&p, size_t &cap, size_t want)
+{
+    if (want <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t n = std::max<size_t>(want + want / 4, 1024);
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), n * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+template <class T>
+hipError_t grow_dev(T *&p, size_t &cap, size_t want, size_t unit)
+{
+    if (want <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t n = std::max<size_t>(want + want / 4, 1024);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), n * unit);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+// Copy with up to 16 host threads (large copies only).
+void parallel_copy(double *dst, const double *src, size_t n)
+{
+    const size_t bytes = n * sizeof(double);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = unsigned(std::min<size_t>({size_t(16), size_t(hw), bytes / (4u << 20) + 1}));
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const size_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        pool.emplace_back([=] { std::memcpy(dst + b, src + b, (e - b) * sizeof(double)); });
+    }
+    for (auto &th : pool) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int sfmm_staging_create(sfmm_context *ctx, sfmm_staging **out)
+{
+    if (!ctx || !out) return SFMM_ERR_LOGIC;
+    *out = nullptr;
+    sfmm_staging *st = new (std::nothrow) sfmm_staging;
+    if (!st) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
+    st->ctx = ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete st;
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        return hip_fail(ctx, e, "staging stream");
+    }
+    *out = st;
+    return SFMM_OK;
+}
+
+void sfmm_staging_destroy(sfmm_staging *st)
+{
+    if (!st) return;
+    (void)hipSetDevice(st->ctx->device);
+    if (st->stream) (void)hipStreamSynchronize(st->stream);
+    if (st->h_src) (void)hipHostFree(st->h_src);
+    if (st->h_dst) (void)hipHostFree(st->h_dst);
+    if (st->d_src) (void)hipFree(st->d_src);
+    if (st->d_dst) (void)hipFree(st->d_dst);
+    if (st->d_plan) (void)hipFree(st->d_plan);
+    if (st->stream) (void)hipStreamDestroy(st->stream);
+    delete st;
+}
+
+int sfmm_staging_reserve(sfmm_staging *st, size_t src_len, size_t dst_len, double **h_src, double **h_dst)
+{
+    if (!st || !h_src || !h_dst) return SFMM_ERR_LOGIC;
+    hipError_t e = hipSetDevice(st->ctx->device);
+    if (e == hipSuccess) e = grow_host(st->h_src, st->h_src_cap, std::max<size_t>(src_len, 2));
+    if (e == hipSuccess) e = grow_host(st->h_dst, st->h_dst_cap, std::max<size_t>(dst_len, 2));
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lock(st->ctx->mu);
+        return fail(st->ctx, SFMM_ERR_ALLOC, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    *h_src = st->h_src;
+    *h_dst = st->h_dst;
+    return SFMM_OK;
+}
+
+int sfmm_staging_translate(sfmm_staging *st, int op, const sfmm_item *items, size_t n, size_t src_len,
+                           size_t dst_len)
+{
+    if (!st) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = st->ctx;
+    if (src_len > st->h_src_cap || dst_len > st->h_dst_cap) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: staging pools not reserved for these lengths");
+    }
+    // Bounds of every block against the pools, before any copy.
+    for (size_t i = 0; i < n; ++i) {
+        const sfmm_item &it = items[i];
+        if (it.src + uint64_t(it.p_in) * (it.p_in + 1) > src_len ||
+            it.dst + uint64_t(it.p_out) * (it.p_out + 1) > dst_len) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: item outside its pool");
+        }
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = grow_dev(st->d_src, st->d_src_cap, std::max<size_t>(src_len, 2), sizeof(double));
+    if (e == hipSuccess) e = grow_dev(st->d_dst, st->d_dst_cap, std::max<size_t>(dst_len, 2), sizeof(double));
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        return fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    // copies in first, so the host-side plan preparation overlaps them
+    e = hipMemcpyAsync(st->d_src, st->h_src, src_len * sizeof(double), hipMemcpyHostToDevice, st->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(st->d_dst, st->h_dst, dst_len * sizeof(double), hipMemcpyHostToDevice, st->stream);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st->stream);
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        return hip_fail(ctx, e, "hipMemcpyAsync");
+    }
+    sfmm_plan *plan = nullptr;
+    int rc = sfmm_plan_prepare(ctx, op, items, n, &plan);
+    if (rc == SFMM_OK && plan->n_items) {
+        const size_t bytes = sfmm_plan_device_bytes(plan);
+        e = grow_dev(reinterpret_cast<unsigned char *&>(st->d_plan), st->d_plan_cap, bytes, 1);
+        if (e != hipSuccess) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            rc = fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        if (rc == SFMM_OK) rc = sfmm_plan_bind(plan, st->d_plan, st->d_plan_cap, st->stream);
+        if (rc == SFMM_OK) rc = sfmm_plan_execute(plan, st->d_src, src_len, st->d_dst, dst_len, st->stream);
+        if (rc == SFMM_OK) {
+            e = hipMemcpyAsync(st->h_dst, st->d_dst, dst_len * sizeof(double), hipMemcpyDeviceToHost, st->stream);
+            if (e != hipSuccess) {
+                std::lock_guard<std::mutex> lock(ctx->mu);
+                rc = hip_fail(ctx, e, "hipMemcpyAsync");
+            }
+        }
+    }
+    // the plan's tables are read by the bind copy: free them only after it
+    e = hipStreamSynchronize(st->stream);
+    if (plan) sfmm_plan_destroy(plan);
+    if (rc == SFMM_OK && e != hipSuccess) {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        rc = hip_fail(ctx, e, "hipStreamSynchronize");
+    }
+    return rc;
+}
+
 int sfmm_translate_host(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, const double *src_pool,
                         size_t src_len, double *dst_pool, size_t dst_len)
 {
     if (!ctx) return SFMM_ERR_LOGIC;
-    // Bounds of every block against the host pools, before any copy.
     for (size_t i = 0; i < n; ++i) {
         const sfmm_item &it = items[i];
         if (it.src + uint64_t(it.p_in) * (it.p_in + 1) > src_len ||
-            it.dst + uint64_t(it.p_out) * (it.p_out + 1) > dst_len)
+            it.dst + uint64_t(it.p_out) * (it.p_out + 1) > dst_len) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
             return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: item outside its pool");
-    }
-    sfmm_plan *plan = nullptr;
-    int rc = sfmm_plan_create(ctx, op, items, n, &plan);
-    if (rc != SFMM_OK) return rc;
-    if (plan->n_items == 0) {
-        sfmm_plan_destroy(plan);
-        return SFMM_OK;
-    }
-    double *d_src = nullptr, *d_dst = nullptr;
-    hipError_t e = hipMalloc(&d_src, std::max<size_t>(src_len, 2) * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&d_dst, std::max<size_t>(dst_len, 2) * sizeof(double));
-    if (e != hipSuccess) {
-        if (d_src) (void)hipFree(d_src);
-        sfmm_plan_destroy(plan);
-        return fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
-    }
-    e = hipMemcpy(d_src, src_pool, src_len * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d_dst, dst_pool, dst_len * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        rc = sfmm_plan_execute(plan, d_src, src_len, d_dst, dst_len, nullptr);
-        if (rc == SFMM_OK) {
-            e = hipMemcpy(dst_pool, d_dst, dst_len * sizeof(double), hipMemcpyDeviceToHost);
-            if (e != hipSuccess) rc = hip_fail(ctx, e, "hipMemcpy");
         }
-    } else {
-        rc = hip_fail(ctx, e, "hipMemcpy");
     }
-    (void)hipFree(d_src);
-    (void)hipFree(d_dst);
-    sfmm_plan_destroy(plan);
+    if (n == 0) return SFMM_OK;
+    // one staging object per context for this blocking convenience; calls
+    // on one context are serialised here
+    std::lock_guard<std::mutex> guard(ctx->host_mu);
+    if (!ctx->host_staging) {
+        int rc = sfmm_staging_create(ctx, &ctx->host_staging);
+        if (rc != SFMM_OK) return rc;
+    }
+    sfmm_staging *st = ctx->host_staging;
+    double *hs, *hd;
+    int rc = sfmm_staging_reserve(st, src_len, dst_len, &hs, &hd);
+    if (rc != SFMM_OK) return rc;
+    parallel_copy(hs, src_pool, src_len);
+    parallel_copy(hd, dst_pool, dst_len);
+    rc = sfmm_staging_translate(st, op, items, n, src_len, dst_len);
+    if (rc == SFMM_OK) parallel_copy(dst_pool, hd, dst_len);
     return rc;
 }
 

@@ -9,9 +9,9 @@
 //   multithreaded_translator         translations.hpp:47-87
 //
 // Translations run on the GPU: the distinct solids of a call are packed into
-// host pools (FP64; float solids are widened), translated by
-// sfmm_translate_host (device pools, one plan, list order per target) and
-// written back.  Vector-valued solids become one item per dimension, as the
+// the pinned host pools of a staging object (FP64; float solids are
+// widened), translated by sfmm_staging_translate (device pools, one plan,
+// list order per target) and written back.  Vector-valued solids become one item per dimension, as the
 // reference queues them (translations.cpp:654-658).
 #include <solidfmm/handles.hpp>
 #include <solidfmm/harmonics.hpp>
@@ -29,6 +29,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -318,13 +319,21 @@ operator_data<real>::~operator_data() { sfmm_destroy(ctx); }
 template class operator_data<float>;
 template class operator_data<double>;
 
-// Host staging reused across calls: the packed pools and the item list.
+// Host staging reused across calls: the item list, the distinct blocks of
+// a call, and a C-ABI staging object (pinned host pools, device pools,
+// stream), created on first use with the operator data's context.
 template <typename real>
 struct threadlocal_buffer<real>::staging {
-    std::vector<double> src, dst;
+    struct block {
+        real *host;     // the solid's data
+        uint64_t off;   // offset in the staging pool (reals)
+        uint64_t len;   // reals
+    };
     std::vector<sfmm_item> items;
     std::unordered_map<const void *, uint64_t> src_off, dst_off;
-    std::vector<std::pair<solid<real> *, uint64_t>> targets;
+    std::vector<block> src_blocks, dst_blocks;
+    sfmm_staging *st = nullptr;
+    ~staging() { sfmm_staging_destroy(st); }
 };
 
 template <typename real>
@@ -472,18 +481,42 @@ void check( int op, const operator_data<real> &od, const threadlocal_buffer<real
     }
 }
 
-// Translate on the GPU; returns an SFMM_* code.
+// Run f(b, e) over [0, n) on up to 16 threads when the blocks move at
+// least `bytes` in total; otherwise on the calling thread.
+template <class F>
+void parallel_blocks( size_t n, size_t bytes, F f )
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = unsigned(std::min<size_t>({ size_t(16), size_t(hw), bytes / (2u << 20) + 1, n }));
+    if (nt <= 1) {
+        f(size_t(0), n);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 1; t < nt; ++t) {
+        const size_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b < e) pool.emplace_back(f, b, e);
+    }
+    f(size_t(0), std::min(n, chunk));
+    for (auto &th : pool) th.join();
+}
+
+// Translate on the GPU; returns an SFMM_* code.  The distinct source and
+// target blocks are packed (in parallel, widened to FP64) straight into the
+// staging object's pinned pools, translated there, and the targets are
+// written back from the pinned target pool.
 template <typename real>
 int run( int op, const operator_data<real> &od, threadlocal_buffer<real> &buf, const translation_info<real> *b,
          const translation_info<real> *e )
 {
     typename threadlocal_buffer<real>::staging &st = *buf.stage();
-    st.src.clear();
-    st.dst.clear();
     st.items.clear();
     st.src_off.clear();
     st.dst_off.clear();
-    st.targets.clear();
+    st.src_blocks.clear();
+    st.dst_blocks.clear();
+    uint64_t src_len = 0, dst_len = 0;
     for (const translation_info<real> *i = b; i != e; ++i) {
         const solid<real> *s = i->source;
         solid<real> *t = i->target;
@@ -492,17 +525,18 @@ int run( int op, const operator_data<real> &od, threadlocal_buffer<real> &buf, c
         auto si = st.src_off.find(s);
         uint64_t so;
         if (si == st.src_off.end()) {
-            so = st.src.size();
+            so = src_len;
             st.src_off.emplace(s, so);
-            st.src.insert(st.src.end(), s->memptr(), s->memptr() + pin * (pin + 1) * dim);
+            st.src_blocks.push_back({ const_cast<real *>(s->memptr()), so, pin * (pin + 1) * dim });
+            src_len += pin * (pin + 1) * dim;
         } else so = si->second;
         auto ti = st.dst_off.find(t);
         uint64_t to;
         if (ti == st.dst_off.end()) {
-            to = st.dst.size();
+            to = dst_len;
             st.dst_off.emplace(t, to);
-            st.dst.insert(st.dst.end(), t->memptr(), t->memptr() + pout * (pout + 1) * dim);
-            st.targets.emplace_back(t, to);
+            st.dst_blocks.push_back({ t->memptr(), to, pout * (pout + 1) * dim });
+            dst_len += pout * (pout + 1) * dim;
         } else to = ti->second;
         for (size_t d = 0; d < dim; ++d) {
             sfmm_item it {};
@@ -517,14 +551,34 @@ int run( int op, const operator_data<real> &od, threadlocal_buffer<real> &buf, c
         }
     }
     if (st.items.empty()) return SFMM_OK;
-    int rc = sfmm_translate_host(od.context(), op, st.items.data(), st.items.size(), st.src.data(),
-                                 st.src.size(), st.dst.data(), st.dst.size());
-    if (rc != SFMM_OK) return rc;
-    for (auto &tg : st.targets) {
-        solid<real> *t = tg.first;
-        const size_t len = t->order() * (t->order() + 1) * t->dimension();
-        for (size_t k = 0; k < len; ++k) t->memptr()[k] = static_cast<real>(st.dst[tg.second + k]);
+    if (!st.st) {
+        const int rc = sfmm_staging_create(od.context(), &st.st);
+        if (rc != SFMM_OK) return rc;
     }
+    double *hs = nullptr, *hd = nullptr;
+    int rc = sfmm_staging_reserve(st.st, src_len, dst_len, &hs, &hd);
+    if (rc != SFMM_OK) return rc;
+    auto pack = [&](const std::vector<typename threadlocal_buffer<real>::staging::block> &blocks, double *pool,
+                    uint64_t len) {
+        parallel_blocks(blocks.size(), len * sizeof(double), [&](size_t bb, size_t be) {
+            for (size_t k = bb; k < be; ++k) {
+                const auto &blk = blocks[k];
+                double *q = pool + blk.off;
+                for (uint64_t j = 0; j < blk.len; ++j) q[j] = static_cast<double>(blk.host[j]);
+            }
+        });
+    };
+    pack(st.src_blocks, hs, src_len);
+    pack(st.dst_blocks, hd, dst_len);
+    rc = sfmm_staging_translate(st.st, op, st.items.data(), st.items.size(), src_len, dst_len);
+    if (rc != SFMM_OK) return rc;
+    parallel_blocks(st.dst_blocks.size(), dst_len * sizeof(double), [&](size_t bb, size_t be) {
+        for (size_t k = bb; k < be; ++k) {
+            const auto &blk = st.dst_blocks[k];
+            const double *q = hd + blk.off;
+            for (uint64_t j = 0; j < blk.len; ++j) blk.host[j] = static_cast<real>(q[j]);
+        }
+    });
     return SFMM_OK;
 }
 

@@ -200,21 +200,53 @@ class FMM:
     def translations(self):
         return int(sum(len(p[1]) for p in self.passes))
 
-    def evaluate(self, stream=None):
+    def evaluate(self, stream=None, mark=None):
+        """Run every pass; `mark(name)`, if given, is called after each phase
+        (P2M, M2M, M2L, L2L, L2P) on the launching thread, e.g. to record
+        events for phase timings."""
         self.M.zero_()
         self.L.zero_()
         self.phi.zero_()
         self.grad.zero_()
+        mark = mark or (lambda name: None)
+        mark("start")
         self.cellset.p2m(self.sorted_points, self.M, stream)
+        mark("P2M")
+        last = None
         for op, plan in self.plans:
+            if last is not None and op != last:
+                mark({M2M: "M2M", M2L: "M2L", L2L: "L2L"}[last])
             if op == M2M:
                 plan.execute(self.M, self.M, stream)
             elif op == M2L:
                 plan.execute(self.M, self.L, stream)
             else:
                 plan.execute(self.L, self.L, stream)
+            last = op
+        if last is not None:
+            mark({M2M: "M2M", M2L: "M2L", L2L: "L2L"}[last])
         self.cellset.l2p(self.sorted_points, self.L, self.phi, self.grad, stream)
+        mark("L2P")
         return self.phi[self.inverse], self.grad[self.inverse]
+
+    def near_field_at(self, points, i):
+        """Host direct sum over the near pairs of point i (caller's order):
+        (potential, gradient).  For checks on samples, not for speed."""
+        tree = self.tree
+        if not hasattr(self, "_leaf_of"):
+            leaf_of = np.empty(self.n, dtype=np.int64)
+            for c in np.nonzero(tree.leaf)[0]:
+                leaf_of[tree.first[c]:tree.first[c] + tree.count[c]] = c
+            self._leaf_of = leaf_of
+            self._rank = np.argsort(tree.order)
+            self._srt = np.asarray(points, dtype=np.float64)[tree.order]
+        near = self.near_src[self.near_dst == self._leaf_of[self._rank[i]]]
+        idx = np.concatenate([np.arange(tree.first[s], tree.first[s] + tree.count[s]) for s in near])
+        d = np.asarray(points[i, :3], dtype=np.float64) - self._srt[idx, :3]
+        r = np.linalg.norm(d, axis=1)
+        r[r == 0] = np.inf
+        q = self._srt[idx, 3]
+        return (q / r).sum(), -(q[:, None] * d / r[:, None] ** 3).sum(0)
 
     def near_pairs(self):
         """Directed near-field leaf pairs (source, target), self pairs included."""
