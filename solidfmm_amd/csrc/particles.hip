@@ -11,7 +11,7 @@
 // (n+m)(n-m) R_n^m = (2n-1) z R_{n-1}^m - |x|^2 R_{n-2}^m; dR is the same
 // recurrence differentiated term by term.
 //
-// P2M: one workgroup per cell, one thread per coefficient (n, m): the
+// P2M: one wave per cell, one lane per coefficient (n, m): the
 // thread walks the cell's particles (a uniform loop) and evaluates its own
 // R_n^m along diagonal m and column m, O(n) per particle, so there is no
 // reduction across threads and the block is written once, coalesced.
@@ -28,7 +28,7 @@
 namespace sfmm {
 namespace {
 
-constexpr int kP2MThreads = 256;
+constexpr int kP2MThreads = 64;
 constexpr int kL2PThreads = 64;
 
 __global__ void __launch_bounds__(kP2MThreads)

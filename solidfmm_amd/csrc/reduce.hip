@@ -21,7 +21,13 @@ namespace sfmm {
 namespace {
 
 constexpr int kWaves = 4;
+constexpr int kBatch = 8;  // scratch loads in flight per lane
 
+// One wave per target.  The items' descriptors are fetched 64 at a time (one
+// per lane) and handed to the whole wave with readlane, and each lane keeps
+// kBatch scratch loads in flight before it adds them, in list order, to its
+// running sum: the latency of the item chain is hidden and the additions
+// happen in exactly the order of the item list.
 __global__ void __launch_bounds__(64 * kWaves)
     k_reduce(const RedTarget *__restrict__ targets, uint32_t n_targets, const RedItem *__restrict__ items,
              const double *__restrict__ scratch, double *__restrict__ dst)
@@ -33,9 +39,33 @@ __global__ void __launch_bounds__(64 * kWaves)
         for (uint32_t c0 = 0; c0 < T.len; c0 += 64) {
             const uint32_t c = c0 + uint32_t(lane);
             double sum = 0.0;
-            for (uint32_t k = 0; k < T.count; ++k) {
-                const RedItem it = items[T.first + k];
-                if (c < it.len) sum += scratch[it.off + c];
+            for (uint32_t k0 = 0; k0 < T.count; k0 += 64) {
+                const uint32_t nk = T.count - k0 < 64u ? T.count - k0 : 64u;
+                // lane j holds the descriptor of item k0 + j
+                uint64_t my_off = 0;
+                uint32_t my_len = 0;
+                if (uint32_t(lane) < nk) {
+                    const RedItem it = items[T.first + k0 + uint32_t(lane)];
+                    my_off = it.off;
+                    my_len = it.len;
+                }
+                const uint32_t off_lo = uint32_t(my_off), off_hi = uint32_t(my_off >> 32);
+                for (uint32_t k = 0; k < nk; k += kBatch) {
+                    double v[kBatch];
+                    bool on[kBatch];
+#pragma unroll
+                    for (int b = 0; b < kBatch; ++b) {
+                        const uint32_t kk = k + uint32_t(b) < nk ? k + uint32_t(b) : nk - 1;
+                        const uint64_t off = (uint64_t(uint32_t(__builtin_amdgcn_readlane(int(off_hi), int(kk)))) << 32) |
+                                             uint32_t(__builtin_amdgcn_readlane(int(off_lo), int(kk)));
+                        const uint32_t len = uint32_t(__builtin_amdgcn_readlane(int(my_len), int(kk)));
+                        on[b] = k + uint32_t(b) < nk && c < len;
+                        v[b] = on[b] ? scratch[off + c] : 0.0;
+                    }
+#pragma unroll
+                    for (int b = 0; b < kBatch; ++b)
+                        if (on[b]) sum += v[b];
+                }
             }
             if (c < T.len) dst[T.dst + c] += sum;
         }
