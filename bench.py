@@ -320,7 +320,8 @@ def serving_mode(args):
         cap_rps = len(reqs) / (sum(gpu_ms) * 1e-3)
         row = {"precision": name, "requests": len(reqs), "translations": total,
                "alone_gpu_ms": sum(gpu_ms), "alone_translations_per_s": total / (sum(gpu_ms) * 1e-3)}
-        server = S.Server(ctx, pools, len(tree), slots=args.slots, budget_bytes=args.budget_mib << 20)
+        cls = S.NativeServer if args.server == "native" else S.Server
+        server = cls(ctx, pools, len(tree), slots=args.slots, budget_bytes=args.budget_mib << 20)
         reqs = S.make_requests(passes, seed)
         row["saturated"] = server.run(reqs, realtime=False)
         reqs = S.make_requests(passes, seed)
@@ -339,7 +340,8 @@ def serving_mode(args):
                                    f"({len(tree)} cells, depth {tree.depth}), orders 20 (root) to 6 (leaves); "
                                    + (f"M2L lists by dual tree walk, theta={args.theta}, smallest-ball centres; "
                                       if args.lists == "mac" else "M2L by same-level V-lists; ")
-                                   + "M2M/M2L/L2L passes cut into requests of U[1k,50k] translations, Poisson "
+                                   + "M2M/M2L/L2L passes cut into requests of U[1k,50k] translations, "
+                                   + f"{args.server} request loop, Poisson "
                                    f"arrivals at {args.load} x one-at-a-time capacity; {args.slots} streams, "
                                    f"{args.budget_mib} MiB device arena for requests in flight",
                        "points": args.points, "cells": int(len(tree)), "depth": int(tree.depth),
@@ -513,6 +515,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--theta", type=float, default=0.6, help="fmm/serving: acceptance parameter of the MAC lists")
+    ap.add_argument("--server", choices=["native", "python"], default="native",
+                    help="serving: the library's native request loop or the Python one")
     ap.add_argument("--lists", choices=["mac", "vlist"], default="mac",
                     help="serving: dual-tree MAC lists or same-level V-lists")
     args = ap.parse_args()

@@ -234,6 +234,42 @@ int sfmm_p2m(sfmm_cellset *set, const double *dev_points, size_t n_points, doubl
 int sfmm_l2p(sfmm_cellset *set, const double *dev_points, size_t n_points, const double *dev_pool, size_t pool_len,
              double *dev_phi, double *dev_grad, void *stream);
 
+/* HIP device of a context (-1 for NULL). */
+int sfmm_device(const sfmm_context *ctx);
+
+/* Continuous-batching server for translation requests (native loop; the
+ * reference has no counterpart: multithreaded_translator,
+ * translations.hpp:47-87, serialises whole batches with a mutex).  The
+ * server keeps the multipole pool (M2M source and target, M2L source) and
+ * the local pool (M2L target, L2L source and target) of `ncells` cells and
+ * runs requests on `slots` streams of its own:
+ *   - sfmm_server_submit copies a request (operator, items, the cells it
+ *     reads and writes) and queues it in arrival order; `workers` host
+ *     threads prepare plans (sfmm_plan_prepare) while the GPU runs others;
+ *   - the oldest waiting request is admitted when it is prepared, a slot is
+ *     free, its device memory fits in the server's arena of `arena_bytes`,
+ *     and it has no hazard with a request in flight (it writes no cell one
+ *     reads or writes; it reads no cell one writes).  Admission is in
+ *     arrival order, so the pools end as if the requests ran one by one;
+ *   - admitted requests are bound into the arena and launched between two
+ *     events; a scheduler thread polls completions.
+ * Times: sfmm_server_request_info gives arrival and admission (host clock)
+ * and completion (GPU clock), in ms since the server's creation, and the
+ * request's own GPU time; rc is its status (a request whose preparation
+ * failed, or that can never fit in the arena, finishes with an error code
+ * and does not run).  counters: max in flight, arena peak and size, polls
+ * blocked on preparation, slot, arena, hazard.  Destroy drains first. */
+typedef struct sfmm_server sfmm_server;
+int sfmm_server_create(sfmm_context *ctx, double *m_pool, size_t m_len, double *l_pool, size_t l_len, size_t ncells,
+                       int slots, size_t arena_bytes, int workers, sfmm_server **sv);
+int sfmm_server_submit(sfmm_server *sv, int op, const sfmm_item *items, size_t n, const uint32_t *src_cells,
+                       size_t n_src, const uint32_t *dst_cells, size_t n_dst, uint64_t *rid);
+int sfmm_server_drain(sfmm_server *sv);
+int sfmm_server_request_info(sfmm_server *sv, uint64_t rid, double *times, int *rc);
+int sfmm_server_counters(sfmm_server *sv, uint64_t *out);
+double sfmm_server_clock_ms(sfmm_server *sv);
+void sfmm_server_destroy(sfmm_server *sv);
+
 /* Host-pool staging for callers whose expansions live in host memory (the
  * drop-in's solid* lists, FFI callers with host arrays).  A staging object
  * owns pinned host pools, device pools, plan memory and a stream, all

@@ -51,6 +51,16 @@ SIGNATURES = {
     "sfmm_plan_prepare": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_plan_bind": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
+    "sfmm_device": (_c.c_int, [_vp]),
+    "sfmm_server_create": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _c.c_size_t, _c.c_int, _c.c_size_t,
+                                      _c.c_int, _c.POINTER(_vp)]),
+    "sfmm_server_submit": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t,
+                                      _c.POINTER(_c.c_uint64)]),
+    "sfmm_server_drain": (_c.c_int, [_vp]),
+    "sfmm_server_request_info": (_c.c_int, [_vp, _c.c_uint64, _vp, _c.POINTER(_c.c_int)]),
+    "sfmm_server_counters": (_c.c_int, [_vp, _vp]),
+    "sfmm_server_clock_ms": (_c.c_double, [_vp]),
+    "sfmm_server_destroy": (None, [_vp]),
     "sfmm_staging_create": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "sfmm_staging_destroy": (None, [_vp]),
     "sfmm_staging_reserve": (_c.c_int, [_vp, _c.c_size_t, _c.c_size_t, _c.POINTER(_vp), _c.POINTER(_vp)]),

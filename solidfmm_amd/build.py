@@ -28,7 +28,7 @@ COMMON = [
 # parallel.
 SOURCES = ["lane_prec_dw.hip", "mfma_m2l_rt.hip", "mfma_m2l.hip", "mfma_m2m_rt.hip", "mfma_l2l_rt.hip", "mfma_m2m.hip",
            "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "reduce.hip", "particles.hip", "translate.hip", "generic.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp",
-           "sphere.cpp"]
+           "sphere.cpp", "server.cpp"]
 
 # Host-only C++ (the drop-in API over the C ABI).
 HOST_ONLY = {"dropin.cpp", "sphere.cpp"}

@@ -408,6 +408,8 @@ void sfmm_destroy(sfmm_context *ctx)
 
 size_t sfmm_order(const sfmm_context *ctx) { return ctx ? ctx->P : 0; }
 
+int sfmm_device(const sfmm_context *ctx) { return ctx ? ctx->device : -1; }
+
 const char *sfmm_last_error(const sfmm_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int sfmm_set_kernels(sfmm_context *ctx, int family)

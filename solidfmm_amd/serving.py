@@ -262,6 +262,92 @@ class Server:
                 "overlap": busy / span if span > 0 else 0.0, "blocked_polls": blocked}
 
 
+class NativeServer:
+    """The same continuous-batching policy run by the library's native loop
+    (sfmm_server_*): preparation threads, admission, launches and completion
+    polling are C++; Python only submits requests at their arrival times.
+    `pools` = (multipole pool, local pool), contiguous float64 device
+    tensors."""
+
+    def __init__(self, ctx, pools, ncells, slots=4, budget_bytes=256 << 20, workers=8):
+        import ctypes
+
+        from . import _native as N
+        self._c = ctypes
+        self._lib = N.load()
+        self._check = N._check
+        self.ctx = ctx
+        self.pools = pools
+        h = ctypes.c_void_p()
+        rc = self._lib.sfmm_server_create(ctx.handle, pools[0].data_ptr(), pools[0].numel(), pools[1].data_ptr(),
+                                          pools[1].numel(), int(ncells), int(slots), int(budget_bytes),
+                                          int(workers), ctypes.byref(h))
+        self._check(rc, ctx.handle, "sfmm_server_create")
+        self._h = h
+
+    def submit(self, r):
+        c = self._c
+        src = np.ascontiguousarray(r.src_cells, dtype=np.uint32)
+        dst = np.ascontiguousarray(r.dst_cells, dtype=np.uint32)
+        rid = c.c_uint64()
+        rc = self._lib.sfmm_server_submit(self._h, r.op, r.items.ctypes.data, len(r.items), src.ctypes.data,
+                                          len(src), dst.ctypes.data, len(dst), c.byref(rid))
+        self._check(rc, self.ctx.handle, "sfmm_server_submit")
+        return rid.value
+
+    def run(self, requests, realtime=True):
+        """Submit `requests` at their arrival times (or all at once), wait
+        for all, and return the statistics of Server.run; latency is
+        completion (GPU clock) minus the scheduled arrival."""
+        c = self._c
+        order = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        base = self._lib.sfmm_server_clock_ms(self._h)
+        t0 = time.perf_counter()
+        rids = []
+        for r in order:
+            if realtime:
+                delay = r.arrival - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+            rids.append((r, self.submit(r)))
+        self._check(self._lib.sfmm_server_drain(self._h), self.ctx.handle, "sfmm_server_drain")
+        wall = time.perf_counter() - t0
+        times = np.zeros(4)
+        status = c.c_int()
+        for r, rid in rids:
+            self._check(self._lib.sfmm_server_request_info(self._h, rid, times.ctypes.data, c.byref(status)),
+                        self.ctx.handle, "sfmm_server_request_info")
+            self._check(status.value, self.ctx.handle, f"request {r.rid}")
+            r.done = times[2] - base
+            r.gpu_ms = times[3]
+            r.admitted = (times[1] - base) * 1e-3
+        cnt = np.zeros(7, dtype=np.uint64)
+        self._lib.sfmm_server_counters(self._h, cnt.ctypes.data)
+        lat = np.array([r.done - 1e3 * r.arrival for r in requests])
+        n = sum(len(r.items) for r in requests)
+        busy = sum(r.gpu_ms for r in requests)
+        span = max(r.done for r in requests) - min(1e3 * r.arrival for r in requests)
+        return {"requests": len(requests), "translations": int(n), "wall_s": wall, "span_ms": span,
+                "translations_per_s": n / (span * 1e-3) if span > 0 else 0.0,
+                "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                               "max": float(lat.max())},
+                "max_inflight": int(cnt[0]), "arena_peak_bytes": int(cnt[1]), "arena_bytes": int(cnt[2]),
+                "sum_request_gpu_ms": busy, "overlap": busy / span if span > 0 else 0.0,
+                "blocked_polls": {"prepare": int(cnt[3]), "slot": int(cnt[4]), "arena": int(cnt[5]),
+                                  "hazard": int(cnt[6])}}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.sfmm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def make_requests(passes, seed, lo=1000, hi=50000):
     """Cut each pass of translation_lists() into requests of U[lo, hi]
     translations, cut at target boundaries (a target's items stay in one

@@ -30,9 +30,14 @@ def sfmm():
     return solidfmm_amd
 
 
-def workload(npts, seed, lo, hi):
-    tree = O.Octree(O.plummer(npts, seed), leaf_size=64)
-    P, off, pool_len, passes = O.translation_lists(tree, 20, 6)
+def workload(npts, seed, lo, hi, lists="vlist"):
+    pts = O.plummer(npts, seed)
+    tree = O.Octree(pts, leaf_size=64)
+    if lists == "mac":
+        from solidfmm_amd import fmm as F
+        P, off, pool_len, passes = F.mac_translation_lists(tree, pts, 0.6, 20, 6)
+    else:
+        P, off, pool_len, passes = O.translation_lists(tree, 20, 6)
     m0 = W.uniform(seed, 7, pool_len, -1.0, 1.0)
     for c in range(len(tree)):  # Im C_n^0 = 0
         p = int(P[c])
@@ -45,16 +50,19 @@ def pools_from(m0, pool_len):
     return (torch.from_numpy(m0.copy()).cuda(), torch.zeros(pool_len, dtype=torch.float64, device="cuda"))
 
 
-@pytest.mark.parametrize("precision", [0, 2])
-def test_alone_equals_mix(sfmm, precision):
-    tree, pool_len, m0, reqs = workload(20000, 5, 300, 3000)
+@pytest.mark.parametrize("precision,kind,lists", [(0, "python", "vlist"), (2, "python", "vlist"),
+                                                   (0, "native", "vlist"), (2, "native", "vlist"),
+                                                   (0, "native", "mac")])
+def test_alone_equals_mix(sfmm, precision, kind, lists):
+    tree, pool_len, m0, reqs = workload(20000, 5, 300, 3000, lists)
     ctx = sfmm.Context(20, precision=precision)
     alone = pools_from(m0, pool_len)
     S.run_sequential(ctx, alone, reqs)
     torch.cuda.synchronize()
     mix = pools_from(m0, pool_len)
     budget = 8 << 20  # a few requests at a time (M2L requests carry scratch)
-    server = S.Server(ctx, mix, len(tree), slots=4, budget_bytes=budget)
+    cls = S.Server if kind == "python" else S.NativeServer
+    server = cls(ctx, mix, len(tree), slots=4, budget_bytes=budget)
     stats = server.run(reqs, realtime=False)
     print(f"precision={precision} requests={len(reqs)} stats={stats}", flush=True)
     for a, b in zip(alone, mix):
@@ -92,12 +100,48 @@ def test_alone_matches_oracle(sfmm, orc):
         assert worst <= 2e-8, worst
 
 
-def test_poisson_run_reports(sfmm):
+@pytest.mark.parametrize("kind", ["python", "native"])
+def test_poisson_run_reports(sfmm, kind):
     tree, pool_len, m0, reqs = workload(20000, 8, 300, 3000)
     ctx = sfmm.Context(20)
     S.poisson_arrivals(reqs, 2000.0, 3)
-    server = S.Server(ctx, pools_from(m0, pool_len), len(tree), slots=4)
+    cls = S.Server if kind == "python" else S.NativeServer
+    server = cls(ctx, pools_from(m0, pool_len), len(tree), slots=4)
     st = server.run(reqs, realtime=True)
     assert st["translations"] == sum(len(r.items) for r in reqs)
     assert 0 < st["latency_ms"]["p50"] <= st["latency_ms"]["p99"] <= st["latency_ms"]["max"]
     assert st["translations_per_s"] > 0
+
+
+def test_native_server_rejects_and_continues(sfmm):
+    """A request with orders above the context's, and one that can never fit
+    in the arena, finish with their error codes; the others still run and
+    match running alone."""
+    import ctypes
+    tree, pool_len, m0, reqs = workload(8000, 9, 300, 2000)
+    ctx = sfmm.Context(20)
+    alone = pools_from(m0, pool_len)
+    S.run_sequential(ctx, alone, reqs)
+    mix = pools_from(m0, pool_len)
+    server = S.NativeServer(ctx, mix, len(tree), slots=2, budget_bytes=4 << 20)
+    bad = S.Request(10 ** 6, W.M2L, reqs[0].items.copy(), reqs[0].src_cells, reqs[0].dst_cells)
+    bad.items["p_in"] = 21
+    rid_bad = server.submit(bad)
+    big = [r for r in reqs if r.op == W.M2L][0]
+    huge = S.Request(10 ** 6 + 1, W.M2L, np.tile(big.items, 400), big.src_cells, big.dst_cells)
+    rid_huge = server.submit(huge)
+    ok = [(r, server.submit(r)) for r in reqs]
+    lib = server._lib
+    assert lib.sfmm_server_drain(server._h) == 0
+    times = np.zeros(4)
+    rc = ctypes.c_int()
+    assert lib.sfmm_server_request_info(server._h, rid_bad, times.ctypes.data, ctypes.byref(rc)) == 0
+    assert rc.value == sfmm.SFMM_ERR_RANGE if hasattr(sfmm, "SFMM_ERR_RANGE") else rc.value == -2
+    assert lib.sfmm_server_request_info(server._h, rid_huge, times.ctypes.data, ctypes.byref(rc)) == 0
+    assert rc.value == -4
+    for r, rid in ok:
+        assert lib.sfmm_server_request_info(server._h, rid, times.ctypes.data, ctypes.byref(rc)) == 0
+        assert rc.value == 0 and times[2] >= 0
+    server.close()
+    for a, b in zip(alone, mix):
+        assert torch.equal(a, b)
