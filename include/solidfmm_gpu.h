@@ -264,6 +264,12 @@ int sfmm_server_create(sfmm_context *ctx, double *m_pool, size_t m_len, double *
                        int slots, size_t arena_bytes, int workers, sfmm_server **sv);
 int sfmm_server_submit(sfmm_server *sv, int op, const sfmm_item *items, size_t n, const uint32_t *src_cells,
                        size_t n_src, const uint32_t *dst_cells, size_t n_dst, uint64_t *rid);
+/* As sfmm_server_submit, but the request arrives at `arrival_ms` on the
+ * server's clock (sfmm_server_clock_ms): it is held, not prepared and not
+ * admitted before then.  A time in the past means now.  For replaying
+ * arrival processes without a host thread timing the submissions. */
+int sfmm_server_submit_at(sfmm_server *sv, int op, const sfmm_item *items, size_t n, const uint32_t *src_cells,
+                          size_t n_src, const uint32_t *dst_cells, size_t n_dst, double arrival_ms, uint64_t *rid);
 int sfmm_server_drain(sfmm_server *sv);
 int sfmm_server_request_info(sfmm_server *sv, uint64_t rid, double *times, int *rc);
 int sfmm_server_counters(sfmm_server *sv, uint64_t *out);

@@ -56,6 +56,8 @@ SIGNATURES = {
                                       _c.c_int, _c.POINTER(_vp)]),
     "sfmm_server_submit": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t,
                                       _c.POINTER(_c.c_uint64)]),
+    "sfmm_server_submit_at": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t,
+                                         _c.c_double, _c.POINTER(_c.c_uint64)]),
     "sfmm_server_drain": (_c.c_int, [_vp]),
     "sfmm_server_request_info": (_c.c_int, [_vp, _c.c_uint64, _vp, _c.POINTER(_c.c_int)]),
     "sfmm_server_counters": (_c.c_int, [_vp, _vp]),

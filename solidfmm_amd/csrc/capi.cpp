@@ -207,6 +207,10 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
     keys.reserve(n);
     FlatMap seen;  // target -> occurrences so far
     uint32_t rounds = 0;
+    // kernel family per (p_in, p_out), filled on first use: lists repeat a
+    // handful of order pairs millions of times
+    constexpr int kNoKernel = -2, kUnknown = -1;
+    std::vector<int8_t> fam_of(size_t(ctx->P + 1) * (ctx->P + 1), int8_t(kUnknown));
     for (size_t i = 0; i < n; ++i) {
         const sfmm_item &it = items[i];
         if (it.p_in > ctx->P || it.p_out > ctx->P)
@@ -215,11 +219,16 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         if ((it.src | it.dst) & 1u)
             return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: expansion offsets must be even (16-byte aligned)");
         if (it.p_in == 0 || it.p_out == 0) continue;
-        int fam = 0;
-        if (!choose(ctx, op, it.p_in, it.p_out, fam))
+        int8_t &fc = fam_of[size_t(it.p_in) * (ctx->P + 1) + it.p_out];
+        if (fc == kUnknown) {
+            int f = 0;
+            fc = int8_t(choose(ctx, op, it.p_in, it.p_out, f) ? f : kNoKernel);
+        }
+        if (fc == kNoKernel)
             return fail(ctx, SFMM_ERR_UNSUPPORTED,
                         std::string("solidfmm: no kernel for ") + op_name(op) + " with p_in=" +
                             std::to_string(it.p_in) + ", p_out=" + std::to_string(it.p_out));
+        const int fam = fc;
         const uint32_t pb = std::max(it.p_in, it.p_out);
         bool fresh;
         const uint32_t round = seen.get(it.dst, 0, fresh)++;

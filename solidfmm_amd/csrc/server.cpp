@@ -133,6 +133,7 @@ struct sfmm_server {
     std::condition_variable cv_done;  // drain: a request finished
     std::deque<Request *> waiting;    // arrival order
     std::deque<Request *> to_prepare;
+    std::deque<Request *> future;     // submitted ahead of their arrival time, by arrival
     std::vector<Request *> inflight;
     std::vector<std::unique_ptr<Request>> all;
     size_t unfinished = 0;
@@ -210,6 +211,16 @@ struct sfmm_server {
         std::unique_lock<std::mutex> lock(mu);
         for (;;) {
             bool progressed = false;
+            // requests whose arrival time has come join the queue
+            const double now = now_ms();
+            while (!future.empty() && future.front()->arrival_ms <= now) {
+                Request *r = future.front();
+                future.pop_front();
+                waiting.push_back(r);
+                to_prepare.push_back(r);
+                cv_prep.notify_one();
+                progressed = true;
+            }
             for (size_t i = 0; i < inflight.size();) {
                 Request &r = *inflight[i];
                 if (hipEventQuery(r.ev_end) == hipSuccess) {
@@ -274,13 +285,17 @@ struct sfmm_server {
                 progressed = true;
             }
             max_inflight = std::max(max_inflight, inflight.size());
-            if (stop && waiting.empty() && inflight.empty()) return;
+            if (stop && waiting.empty() && inflight.empty() && future.empty()) return;
             if (progressed) continue;
             if (!inflight.empty()) {
                 // kernels are running: poll again shortly
                 lock.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(5));
                 lock.lock();
+            } else if (!future.empty()) {
+                const double wait_ms = future.front()->arrival_ms - now_ms();
+                if (wait_ms > 0)
+                    cv_work.wait_for(lock, std::chrono::duration<double, std::milli>(std::min(wait_ms, 1.0)));
             } else {
                 cv_work.wait(lock);
             }
@@ -359,6 +374,12 @@ int sfmm_server_create(sfmm_context *ctx, double *m_pool, size_t m_len, double *
 int sfmm_server_submit(sfmm_server *sv, int op, const sfmm_item *items, size_t n, const uint32_t *src_cells,
                        size_t n_src, const uint32_t *dst_cells, size_t n_dst, uint64_t *rid)
 {
+    return sfmm_server_submit_at(sv, op, items, n, src_cells, n_src, dst_cells, n_dst, -1.0, rid);
+}
+
+int sfmm_server_submit_at(sfmm_server *sv, int op, const sfmm_item *items, size_t n, const uint32_t *src_cells,
+                          size_t n_src, const uint32_t *dst_cells, size_t n_dst, double arrival_ms, uint64_t *rid)
+{
     if (!sv || !rid || op < SFMM_M2M || op > SFMM_L2L) return SFMM_ERR_LOGIC;
     if ((n && !items) || (n_src && !src_cells) || (n_dst && !dst_cells)) return SFMM_ERR_LOGIC;
     auto r = std::unique_ptr<Request>(new (std::nothrow) Request);
@@ -374,15 +395,23 @@ int sfmm_server_submit(sfmm_server *sv, int op, const sfmm_item *items, size_t n
     }
     std::lock_guard<std::mutex> lock(sv->mu);
     if (sv->stop) return SFMM_ERR_LOGIC;
+    const double now = sv->now_ms();
     r->id = sv->all.size();
-    r->arrival_ms = sv->now_ms();
+    r->arrival_ms = arrival_ms > now ? arrival_ms : now;
     *rid = r->id;
     Request *p = r.get();
     sv->all.push_back(std::move(r));
     ++sv->unfinished;
-    sv->waiting.push_back(p);
-    sv->to_prepare.push_back(p);
-    sv->cv_prep.notify_one();
+    if (arrival_ms > now) {
+        // held until its arrival time; the queue stays in arrival order
+        auto pos = std::upper_bound(sv->future.begin(), sv->future.end(), p,
+                                    [](const Request *a, const Request *b) { return a->arrival_ms < b->arrival_ms; });
+        sv->future.insert(pos, p);
+    } else {
+        sv->waiting.push_back(p);
+        sv->to_prepare.push_back(p);
+        sv->cv_prep.notify_one();
+    }
     sv->cv_work.notify_one();
     return SFMM_OK;
 }

@@ -269,8 +269,9 @@ class NativeServer:
     `pools` = (multipole pool, local pool), contiguous float64 device
     tensors."""
 
-    def __init__(self, ctx, pools, ncells, slots=4, budget_bytes=256 << 20, workers=8):
+    def __init__(self, ctx, pools, ncells, slots=4, budget_bytes=256 << 20, workers=None):
         import ctypes
+        import os
 
         from . import _native as N
         self._c = ctypes
@@ -281,35 +282,35 @@ class NativeServer:
         h = ctypes.c_void_p()
         rc = self._lib.sfmm_server_create(ctx.handle, pools[0].data_ptr(), pools[0].numel(), pools[1].data_ptr(),
                                           pools[1].numel(), int(ncells), int(slots), int(budget_bytes),
-                                          int(workers), ctypes.byref(h))
+                                          int(workers or min(16, os.cpu_count() or 1)), ctypes.byref(h))
         self._check(rc, ctx.handle, "sfmm_server_create")
         self._h = h
 
-    def submit(self, r):
+    def submit(self, r, arrival_ms=-1.0):
+        """Submit request r now, or held until `arrival_ms` on the server's
+        clock; returns its id."""
         c = self._c
         src = np.ascontiguousarray(r.src_cells, dtype=np.uint32)
         dst = np.ascontiguousarray(r.dst_cells, dtype=np.uint32)
         rid = c.c_uint64()
-        rc = self._lib.sfmm_server_submit(self._h, r.op, r.items.ctypes.data, len(r.items), src.ctypes.data,
-                                          len(src), dst.ctypes.data, len(dst), c.byref(rid))
-        self._check(rc, self.ctx.handle, "sfmm_server_submit")
+        rc = self._lib.sfmm_server_submit_at(self._h, r.op, r.items.ctypes.data, len(r.items), src.ctypes.data,
+                                             len(src), dst.ctypes.data, len(dst), float(arrival_ms), c.byref(rid))
+        self._check(rc, self.ctx.handle, "sfmm_server_submit_at")
         return rid.value
 
-    def run(self, requests, realtime=True):
-        """Submit `requests` at their arrival times (or all at once), wait
-        for all, and return the statistics of Server.run; latency is
+    def run(self, requests, realtime=True, lead_ms=None):
+        """Replay `requests`: every request is handed to the server up front,
+        held until base + its arrival time (or base for all, realtime=False),
+        where base lies `lead_ms` ahead so that the submissions finish first;
+        wait for all and return the statistics of Server.run.  Latency is
         completion (GPU clock) minus the scheduled arrival."""
         c = self._c
         order = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        base = self._lib.sfmm_server_clock_ms(self._h)
+        if lead_ms is None:
+            lead_ms = 50.0 + 0.5 * len(order)
+        base = self._lib.sfmm_server_clock_ms(self._h) + lead_ms
+        rids = [(r, self.submit(r, base + (1e3 * r.arrival if realtime else 0.0))) for r in order]
         t0 = time.perf_counter()
-        rids = []
-        for r in order:
-            if realtime:
-                delay = r.arrival - (time.perf_counter() - t0)
-                if delay > 0:
-                    time.sleep(delay)
-            rids.append((r, self.submit(r)))
         self._check(self._lib.sfmm_server_drain(self._h), self.ctx.handle, "sfmm_server_drain")
         wall = time.perf_counter() - t0
         times = np.zeros(4)
@@ -323,10 +324,11 @@ class NativeServer:
             r.admitted = (times[1] - base) * 1e-3
         cnt = np.zeros(7, dtype=np.uint64)
         self._lib.sfmm_server_counters(self._h, cnt.ctypes.data)
-        lat = np.array([r.done - 1e3 * r.arrival for r in requests])
+        arr = {id(r): (1e3 * r.arrival if realtime else 0.0) for r in requests}
+        lat = np.array([r.done - arr[id(r)] for r in requests])
         n = sum(len(r.items) for r in requests)
         busy = sum(r.gpu_ms for r in requests)
-        span = max(r.done for r in requests) - min(1e3 * r.arrival for r in requests)
+        span = max(r.done for r in requests) - min(arr.values())
         return {"requests": len(requests), "translations": int(n), "wall_s": wall, "span_ms": span,
                 "translations_per_s": n / (span * 1e-3) if span > 0 else 0.0,
                 "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
