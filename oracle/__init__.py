@@ -27,10 +27,15 @@ _fp = ctypes.POINTER(ctypes.c_float)
 
 
 def build(reference=True):
-    """Compile liboracle.so, and the reference library when its tree is present."""
+    """Compile liboracle.so, and, when the reference tree is present, the
+    reference library and the programs that run the reference's public API
+    through both libraries (Makefile target dropin; needs the GPU library
+    built first)."""
     targets = ["oracle"]
     if reference and os.path.isdir(os.path.join(REF_TREE, "solidfmm")):
         targets.append("ref")
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "solidfmm_amd", "lib", "libsolidfmm_gpu.so")):
+            targets.append("dropin")
     subprocess.run(["make", "-s", "-C", HERE, "-j8"] + targets, check=True)
 
 
