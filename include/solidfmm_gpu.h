@@ -200,37 +200,42 @@ int sfmm_swap_block(int n, int part, double *out);
 
 /* Particle-side operators, batched over cells (the steps on either side of
  * the translations; in the reference user code over harmonics::R / dR and
- * solid::fmadd / dot, harmonics.cpp:31-174, solid.cpp:40-105).  Points are
- * a device array of (x, y, z, q) quadruples; a cell names its expansion
- * centre, its points [first, first + count) and its expansion block (offset
- * in reals, order) in a pool.  48 bytes. */
+ * solid::fmadd / dot, harmonics.cpp:31-174, solid.cpp:40-105).  A cell
+ * names its expansion centre, its points [first, first + count) and its
+ * expansion block (offset in reals, order) in a pool.  48 bytes.
+ * Vector-valued expansions (dim D = 1..4, as solid<real>(P, dim),
+ * solid.hpp:28-66): the block holds D consecutive P(P+1)-real blocks, and
+ * every point is a row of 3 + D doubles (x, y, z, q_0 .. q_{D-1}); D = 1
+ * gives (x, y, z, q) quadruples.  All cells of a set share D. */
 typedef struct sfmm_cell {
     double x, y, z;   /* expansion centre */
     uint64_t offset;  /* block offset (reals) in the pool, even */
     uint32_t first;   /* first point */
     uint32_t count;   /* number of points */
     uint32_t order;   /* expansion order P (degrees 0..P-1) */
-    uint32_t pad;
+    uint32_t dim;     /* D, 0 meaning 1 */
 } sfmm_cell;
 
 typedef struct sfmm_cellset sfmm_cellset;
 
 /* Validated, device-resident cell list (like a plan, reusable every step):
  * orders <= the context's order (SFMM_ERR_RANGE), point ranges within
- * n_points and even offsets (SFMM_ERR_LOGIC). */
+ * n_points, even offsets and one dim in 1..4 for all cells
+ * (SFMM_ERR_LOGIC). */
 int sfmm_cellset_create(sfmm_context *ctx, const sfmm_cell *host_cells, size_t n_cells, size_t n_points,
                         sfmm_cellset **set);
 void sfmm_cellset_destroy(sfmm_cellset *set);
 
-/* P2M: pool block of cell c += sum_i q_i R(P_c, p_i - centre_c)
- * (tests/test_m2l.cpp:70 forms M = R(P, x_s - x_a) for a unit charge). */
+/* P2M: block d of cell c += sum_i q_{i,d} R(P_c, p_i - centre_c)
+ * (tests/test_m2l.cpp:70 forms M = R(P, x_s - x_a) for a unit charge).
+ * n_points counts points (rows of 3 + D doubles). */
 int sfmm_p2m(sfmm_cellset *set, const double *dev_points, size_t n_points, double *dev_pool, size_t pool_len,
              void *stream);
 
-/* L2P: phi[i] += dot(L_c, R(P_c, p_i - centre_c)) and, if dev_grad is not
- * NULL, grad[3i..3i+2] += dot(L_c, dR(P_c, p_i - centre_c))
- * (tests/test_m2l.cpp:112-116).  phi holds n_points reals and grad
- * 3 n_points (x, y, z per point). */
+/* L2P: phi[i D + d] += dot(L_{c,d}, R(P_c, p_i - centre_c)) and, if
+ * dev_grad is not NULL, grad[3 (i D + d) + k] += dot(L_{c,d}, d/dx_k R(...))
+ * (tests/test_m2l.cpp:112-116).  phi holds D n_points reals and grad
+ * 3 D n_points. */
 int sfmm_l2p(sfmm_cellset *set, const double *dev_points, size_t n_points, const double *dev_pool, size_t pool_len,
              double *dev_phi, double *dev_grad, void *stream);
 

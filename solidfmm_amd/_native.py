@@ -332,7 +332,7 @@ class Plan:
 
 
 CELL_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("offset", "<u8"), ("first", "<u4"),
-                       ("count", "<u4"), ("order", "<u4"), ("pad", "<u4")])
+                       ("count", "<u4"), ("order", "<u4"), ("dim", "<u4")])
 assert CELL_DTYPE.itemsize == 48
 
 
@@ -342,16 +342,19 @@ class CellSet:
     def __init__(self, ctx, cells, n_points):
         cells = np.ascontiguousarray(cells, dtype=CELL_DTYPE)
         self.ctx = ctx
+        self.dim = int(max(cells["dim"][0], 1)) if len(cells) else 1
         h = ctypes.c_void_p()
         rc = load().sfmm_cellset_create(ctx.handle, cells.ctypes.data, len(cells), n_points, ctypes.byref(h))
         _check(rc, ctx.handle, "sfmm_cellset_create")
         self._h = h
 
     def p2m(self, points, pool, stream=None):
-        """pool += P2M of points ((n, 4) float64 device tensor: x, y, z, q)."""
+        """pool += P2M of points ((n, 3 + dim) float64 device tensor: x, y, z,
+        then the dim charges)."""
         pp, pn = _dev_ptr(points)
         mp, mn = _dev_ptr(pool)
-        _check(load().sfmm_p2m(self._h, pp, pn // 4, mp, mn, _stream_ptr(stream)), self.ctx.handle, "sfmm_p2m")
+        _check(load().sfmm_p2m(self._h, pp, pn // (3 + self.dim), mp, mn, _stream_ptr(stream)), self.ctx.handle,
+               "sfmm_p2m")
 
     def l2p(self, points, pool, phi, grad=None, stream=None):
         """phi += potential, grad += gradient of the local expansions at the points."""
@@ -359,9 +362,10 @@ class CellSet:
         lp, ln = _dev_ptr(pool)
         fp, fn = _dev_ptr(phi)
         gp, gn = _dev_ptr(grad) if grad is not None else (None, 3 * fn)
-        if fn < pn // 4 or gn < 3 * (pn // 4):
-            raise LogicError("sfmm_l2p: phi needs one real and grad three reals per point")
-        _check(load().sfmm_l2p(self._h, pp, pn // 4, lp, ln, fp, gp, _stream_ptr(stream)), self.ctx.handle, "sfmm_l2p")
+        npts = pn // (3 + self.dim)
+        if fn < self.dim * npts or gn < 3 * self.dim * npts:
+            raise LogicError("sfmm_l2p: phi needs dim reals and grad 3 dim reals per point")
+        _check(load().sfmm_l2p(self._h, pp, npts, lp, ln, fp, gp, _stream_ptr(stream)), self.ctx.handle, "sfmm_l2p")
 
     def close(self):
         if getattr(self, "_h", None):

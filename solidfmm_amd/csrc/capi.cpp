@@ -983,6 +983,7 @@ struct sfmm_cellset {
     sfmm_context *ctx = nullptr;
     sfmm_cell *d_cells = nullptr;
     size_t n_cells = 0, n_points = 0, pool_extent = 0;
+    int dim = 1;
 };
 
 int sfmm_cellset_create(sfmm_context *ctx, const sfmm_cell *cells, size_t n, size_t n_points, sfmm_cellset **out)
@@ -992,12 +993,16 @@ int sfmm_cellset_create(sfmm_context *ctx, const sfmm_cell *cells, size_t n, siz
     *out = nullptr;
     if (n && !cells) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null cell list");
     size_t extent = 0;
+    const uint32_t dim = n ? std::max(cells[0].dim, 1u) : 1u;
+    if (dim > uint32_t(sfmm::kMaxParticleDim))
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: particle expansions of dimension 1..4 only");
     for (size_t i = 0; i < n; ++i) {
         const sfmm_cell &c = cells[i];
         if (c.order > ctx->P) return fail(ctx, SFMM_ERR_RANGE, "solidfmm: cell order exceeding operator data.");
         if (c.offset & 1u) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: expansion offsets must be even");
         if (uint64_t(c.first) + c.count > n_points) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: cell points out of range");
-        extent = std::max<size_t>(extent, c.offset + size_t(c.order) * (c.order + 1));
+        if (std::max(c.dim, 1u) != dim) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: cells of one set must share dim");
+        extent = std::max<size_t>(extent, c.offset + size_t(dim) * c.order * (c.order + 1));
     }
     sfmm_cellset *set = new (std::nothrow) sfmm_cellset;
     if (!set) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: out of host memory");
@@ -1005,6 +1010,7 @@ int sfmm_cellset_create(sfmm_context *ctx, const sfmm_cell *cells, size_t n, siz
     set->n_cells = n;
     set->n_points = n_points;
     set->pool_extent = extent;
+    set->dim = int(dim);
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess && n) {
         e = hipMalloc(&set->d_cells, n * sizeof(sfmm_cell));
@@ -1035,7 +1041,8 @@ int sfmm_p2m(sfmm_cellset *set, const double *points, size_t n_points, double *p
         return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: points or pool shorter than the cell set needs");
     if (set->n_cells && (!points || !pool)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pointer");
     hipError_t e = hipSetDevice(ctx->device);
-    if (e == hipSuccess) e = sfmm::launch_p2m(points, set->d_cells, set->n_cells, pool, static_cast<hipStream_t>(stream));
+    if (e == hipSuccess)
+        e = sfmm::launch_p2m(points, set->d_cells, set->n_cells, pool, set->dim, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(ctx, e, "p2m launch");
     return SFMM_OK;
 }
@@ -1051,7 +1058,8 @@ int sfmm_l2p(sfmm_cellset *set, const double *points, size_t n_points, const dou
     if (set->n_cells && (!points || !pool || !phi)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pointer");
     hipError_t e = hipSetDevice(ctx->device);
     if (e == hipSuccess)
-        e = sfmm::launch_l2p(points, set->d_cells, set->n_cells, pool, phi, grad, static_cast<hipStream_t>(stream));
+        e = sfmm::launch_l2p(points, set->d_cells, set->n_cells, pool, phi, grad, set->dim,
+                             static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(ctx, e, "l2p launch");
     return SFMM_OK;
 }

@@ -1,7 +1,8 @@
 // Particle-side operators on either side of the translations, batched over
-// cells:
-//   P2M  M_c += sum_{i in c} q_i R(P_c, p_i - c)
-//   L2P  phi_i += dot(L_c, R(P_c, p_i - c)),  grad_i += dot(L_c, dR(...))
+// cells, for D-valued expansions (D blocks per cell, D charges per point):
+//   P2M  M_{c,d} += sum_{i in c} q_{i,d} R(P_c, p_i - c)
+//   L2P  phi_{i,d} += dot(L_{c,d}, R(P_c, p_i - c)),
+//        grad_{i,d} += dot(L_{c,d}, dR(...))
 // In the reference these are user code over harmonics::R / dR and
 // solid::fmadd / dot (harmonics.cpp:31-174, solid.cpp:40-105, the pattern of
 // tests/test_m2l.cpp:70-116).
@@ -31,10 +32,12 @@ namespace {
 constexpr int kP2MThreads = 64;
 constexpr int kL2PThreads = 64;
 
+template <int D>
 __global__ void __launch_bounds__(kP2MThreads)
     k_p2m(const double *__restrict__ pts, const sfmm_cell *__restrict__ cells, uint32_t n_cells,
           double *__restrict__ pool, int acc)
 {
+    constexpr int W = 3 + D;  // doubles per point
     for (uint32_t c = blockIdx.x; c < n_cells; c += gridDim.x) {
         const sfmm_cell cell = cells[c];
         const int P = int(cell.order);
@@ -45,10 +48,11 @@ __global__ void __launch_bounds__(kP2MThreads)
             while (n * (n + 1) / 2 > k) --n;
             while ((n + 1) * (n + 2) / 2 <= k) ++n;
             const int m = k - n * (n + 1) / 2;
-            double sr = 0.0, si = 0.0;
+            double sr[D], si[D];
+            for (int d = 0; d < D; ++d) sr[d] = si[d] = 0.0;
             for (uint32_t i = 0; i < cell.count; ++i) {
-                const double *p = pts + 4 * size_t(cell.first + i);
-                const double x = p[0] - cell.x, y = p[1] - cell.y, z = p[2] - cell.z, q = p[3];
+                const double *p = pts + W * size_t(cell.first + i);
+                const double x = p[0] - cell.x, y = p[1] - cell.y, z = p[2] - cell.z;
                 // diagonal: R_0^0 = 1, R_1^1 = (x + iy)/2, R_j^j = R_{j-1}^{j-1} (x + iy)/(2j)
                 double dr = 1.0, di = 0.0;
                 if (m >= 1) {
@@ -77,20 +81,27 @@ __global__ void __launch_bounds__(kP2MThreads)
                         ai = ti;
                     }
                 }
-                sr = fma(q, ar, sr);
-                si = fma(q, ai, si);
+                for (int d = 0; d < D; ++d) {
+                    sr[d] = fma(p[3 + d], ar, sr[d]);
+                    si[d] = fma(p[3 + d], ai, si[d]);
+                }
             }
             const int kr = n * (n + 1) + m, ki = (n + 1) * (n + 1) + m;
-            M[kr] = (acc ? M[kr] : 0.0) + sr;
-            M[ki] = (acc ? M[ki] : 0.0) + si;
+            for (int d = 0; d < D; ++d) {
+                double *Md = M + d * P * (P + 1);
+                Md[kr] = (acc ? Md[kr] : 0.0) + sr[d];
+                Md[ki] = (acc ? Md[ki] : 0.0) + si[d];
+            }
         }
     }
 }
 
+template <int D>
 __global__ void __launch_bounds__(kL2PThreads)
     k_l2p(const double *__restrict__ pts, const sfmm_cell *__restrict__ cells, uint32_t n_cells,
           const double *__restrict__ pool, double *__restrict__ phi, double *__restrict__ grad)
 {
+    constexpr int W = 3 + D;  // doubles per point
     for (uint32_t c = blockIdx.x; c < n_cells; c += gridDim.x) {
         const sfmm_cell cell = cells[c];
         const int P = int(cell.order);
@@ -98,12 +109,13 @@ __global__ void __launch_bounds__(kL2PThreads)
         for (uint32_t i0 = 0; i0 < cell.count; i0 += kL2PThreads) {
             const uint32_t i = i0 + threadIdx.x;
             const bool valid = i < cell.count;
-            const double *p = pts + 4 * size_t(cell.first + (valid ? i : 0));
+            const double *p = pts + W * size_t(cell.first + (valid ? i : 0));
             const double x = p[0] - cell.x, y = p[1] - cell.y, z = p[2] - cell.z;
             const double r2 = x * x + y * y + z * z;
             // running diagonal value and gradient: R_m^m, d/dx, d/dy, d/dz
             double dr = 1.0, di = 0.0, gdr[3] = {0.0, 0.0, 0.0}, gdi[3] = {0.0, 0.0, 0.0};
-            double f0 = 0.0, g[3] = {0.0, 0.0, 0.0};
+            double f0[D], g[D][3];
+            for (int d = 0; d < D; ++d) f0[d] = g[d][0] = g[d][1] = g[d][2] = 0.0;
             for (int m = 0; m < P; ++m) {
                 if (m == 1) {
                     dr = x / 2;
@@ -168,18 +180,23 @@ __global__ void __launch_bounds__(kL2PThreads)
                             gai[d] = tgi[d] * f;
                         }
                     }
-                    const double lr = L[n * (n + 1) + m], li = L[(n + 1) * (n + 1) + m];
-                    f0 = fma(w * lr, ar, fma(w * li, ai, f0));
-                    for (int d = 0; d < 3; ++d) g[d] = fma(w * lr, gar[d], fma(w * li, gai[d], g[d]));
+                    for (int d = 0; d < D; ++d) {
+                        const double *Ld = L + d * P * (P + 1);
+                        const double lr = Ld[n * (n + 1) + m], li = Ld[(n + 1) * (n + 1) + m];
+                        f0[d] = fma(w * lr, ar, fma(w * li, ai, f0[d]));
+                        for (int k = 0; k < 3; ++k) g[d][k] = fma(w * lr, gar[k], fma(w * li, gai[k], g[d][k]));
+                    }
                 }
             }
             if (valid) {
                 const size_t j = cell.first + i;
-                phi[j] += f0;
-                if (grad) {
-                    grad[3 * j] += g[0];
-                    grad[3 * j + 1] += g[1];
-                    grad[3 * j + 2] += g[2];
+                for (int d = 0; d < D; ++d) {
+                    phi[j * D + d] += f0[d];
+                    if (grad) {
+                        grad[3 * (j * D + d)] += g[d][0];
+                        grad[3 * (j * D + d) + 1] += g[d][1];
+                        grad[3 * (j * D + d) + 2] += g[d][2];
+                    }
                 }
             }
         }
@@ -188,24 +205,54 @@ __global__ void __launch_bounds__(kL2PThreads)
 
 }  // namespace
 
-hipError_t launch_p2m(const double *pts, const sfmm_cell *cells, size_t n_cells, double *pool, hipStream_t stream)
+namespace {
+
+template <int D>
+hipError_t p2m_d(const double *pts, const sfmm_cell *cells, size_t n_cells, double *pool, hipStream_t stream)
 {
-    if (n_cells == 0) return hipSuccess;
-    if (n_cells > 0xFFFFFFFFull) return hipErrorInvalidValue;
-    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_p2m), kP2MThreads, n_cells);
-    hipLaunchKernelGGL(k_p2m, dim3(grid), dim3(kP2MThreads), 0, stream, pts, cells, uint32_t(n_cells), pool, 1);
+    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_p2m<D>), kP2MThreads, n_cells);
+    hipLaunchKernelGGL(k_p2m<D>, dim3(grid), dim3(kP2MThreads), 0, stream, pts, cells, uint32_t(n_cells), pool, 1);
     return hipGetLastError();
 }
 
-hipError_t launch_l2p(const double *pts, const sfmm_cell *cells, size_t n_cells, const double *pool, double *phi,
-                      double *grad, hipStream_t stream)
+template <int D>
+hipError_t l2p_d(const double *pts, const sfmm_cell *cells, size_t n_cells, const double *pool, double *phi,
+                 double *grad, hipStream_t stream)
+{
+    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_l2p<D>), kL2PThreads, n_cells);
+    hipLaunchKernelGGL(k_l2p<D>, dim3(grid), dim3(kL2PThreads), 0, stream, pts, cells, uint32_t(n_cells), pool, phi,
+                       grad);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_p2m(const double *pts, const sfmm_cell *cells, size_t n_cells, double *pool, int dim,
+                      hipStream_t stream)
 {
     if (n_cells == 0) return hipSuccess;
     if (n_cells > 0xFFFFFFFFull) return hipErrorInvalidValue;
-    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_l2p), kL2PThreads, n_cells);
-    hipLaunchKernelGGL(k_l2p, dim3(grid), dim3(kL2PThreads), 0, stream, pts, cells, uint32_t(n_cells), pool, phi,
-                       grad);
-    return hipGetLastError();
+    switch (dim) {
+    case 1: return p2m_d<1>(pts, cells, n_cells, pool, stream);
+    case 2: return p2m_d<2>(pts, cells, n_cells, pool, stream);
+    case 3: return p2m_d<3>(pts, cells, n_cells, pool, stream);
+    case 4: return p2m_d<4>(pts, cells, n_cells, pool, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_l2p(const double *pts, const sfmm_cell *cells, size_t n_cells, const double *pool, double *phi,
+                      double *grad, int dim, hipStream_t stream)
+{
+    if (n_cells == 0) return hipSuccess;
+    if (n_cells > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    switch (dim) {
+    case 1: return l2p_d<1>(pts, cells, n_cells, pool, phi, grad, stream);
+    case 2: return l2p_d<2>(pts, cells, n_cells, pool, phi, grad, stream);
+    case 3: return l2p_d<3>(pts, cells, n_cells, pool, phi, grad, stream);
+    case 4: return l2p_d<4>(pts, cells, n_cells, pool, phi, grad, stream);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace sfmm

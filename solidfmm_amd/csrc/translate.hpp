@@ -101,9 +101,12 @@ hipError_t launch_generic(int op, uint32_t pc, const sfmm_item *items, size_t n,
                           hipStream_t stream, int acc = 1);
 
 // Particle-side operators over device-resident cells (particles.hip).
-hipError_t launch_p2m(const double *pts, const sfmm_cell *cells, size_t n_cells, double *pool, hipStream_t stream);
+// dim: values per point and blocks per cell, 1..kMaxParticleDim.
+inline constexpr int kMaxParticleDim = 4;
+hipError_t launch_p2m(const double *pts, const sfmm_cell *cells, size_t n_cells, double *pool, int dim,
+                      hipStream_t stream);
 hipError_t launch_l2p(const double *pts, const sfmm_cell *cells, size_t n_cells, const double *pool, double *phi,
-                      double *grad, hipStream_t stream);
+                      double *grad, int dim, hipStream_t stream);
 
 // Enqueue dst += T(src) for n items all of orders (p_in, p_out) with
 // pairwise distinct targets, in `family` (kAuto: default_family).

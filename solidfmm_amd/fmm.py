@@ -111,29 +111,33 @@ def cell_orders(tree, p_root, p_leaf):
     return tree.orders(p_root, p_leaf)
 
 
-def fmm_passes(tree, balls, orders, m2l_src, m2l_dst):
+def fmm_passes(tree, balls, orders, m2l_src, m2l_dst, dim=1):
     """(offsets, pool_len, passes) like octree.translation_lists, with the
     expansion centres at the cells' balls and the M2L items of the MAC
     lists.  passes: [(op, items, src, dst, level)], M2M bottom-up, M2L, L2L
-    top-down; items of a pass sorted by target."""
+    top-down; items of a pass sorted by target.  dim > 1: every cell holds
+    dim blocks (solid<real>(P, dim)) and every cell pair gives one item per
+    dimension, as the reference queues them (translations.cpp:654-658)."""
     P = np.asarray(orders, dtype=np.int64)
     size = P * (P + 1)
-    off = np.concatenate([[0], np.cumsum(size)[:-1]]).astype(np.int64)
-    pool_len = int(size.sum())
+    off = np.concatenate([[0], np.cumsum(dim * size)[:-1]]).astype(np.int64)
+    pool_len = int(dim * size.sum())
     ctr = balls[:, :3]
     child = np.nonzero(tree.parent >= 0)[0]
     par = tree.parent[child]
 
     def items_for(src, dst):
-        it = np.zeros(len(src), dtype=ITEM_DTYPE)
-        it["src"] = off[src]
-        it["dst"] = off[dst]
-        it["p_in"] = P[src]
-        it["p_out"] = P[dst]
-        sh = ctr[dst] - ctr[src]
-        it["x"], it["y"], it["z"] = sh[:, 0], sh[:, 1], sh[:, 2]
         o = np.argsort(dst, kind="stable")
-        return it[o], src[o], dst[o]
+        src, dst = src[o], dst[o]
+        it = np.zeros((len(src), dim), dtype=ITEM_DTYPE)
+        d = np.arange(dim)[None, :]
+        it["src"] = off[src][:, None] + d * size[src][:, None]
+        it["dst"] = off[dst][:, None] + d * size[dst][:, None]
+        it["p_in"] = P[src][:, None]
+        it["p_out"] = P[dst][:, None]
+        sh = ctr[dst] - ctr[src]
+        it["x"], it["y"], it["z"] = sh[:, 0, None], sh[:, 1, None], sh[:, 2, None]
+        return it.reshape(-1), np.repeat(src, dim), np.repeat(dst, dim)
 
     passes = []
     for lev in range(tree.depth, 0, -1):
@@ -157,26 +161,30 @@ def mac_translation_lists(tree, points, theta=0.5, p_root=20, p_leaf=6):
 
 
 class FMM:
-    """Far field of the potential sum_j q_j / |x - x_j| and its gradient at
-    every point, on one GPU.
+    """Far field of the potentials phi_d(x) = sum_j q_{j,d} / |x - x_j| and
+    their gradients at every point, on one GPU.
 
-    points: (n, 4) array of x, y, z, q.  The tree, balls, lists, plans and
-    cell sets are built once; `evaluate()` runs P2M, M2M, M2L, L2L and L2P
-    and returns (phi, grad) as device tensors in the caller's point order,
-    holding the contributions of every source outside the point's near
-    pairs."""
+    points: (n, 3 + dim) array of x, y, z and the dim charges (dim 1..4; a
+    vortex method's vector potential has dim 3).  The tree, balls, lists,
+    plans and cell sets are built once; `evaluate()` runs P2M, M2M, M2L, L2L
+    and L2P and returns (phi, grad) as device tensors in the caller's point
+    order, of shapes (n,) and (n, 3) for dim 1, else (n, dim) and
+    (n, dim, 3), holding the contributions of every source outside the
+    point's near pairs."""
 
     def __init__(self, ctx, points, leaf_size=64, theta=0.5, p_root=None, p_leaf=None):
         import torch
         pts = np.ascontiguousarray(points, dtype=np.float64)
         self.n = len(pts)
+        self.dim = dim = pts.shape[1] - 3
         self.tree = tree = Octree(pts[:, :3], leaf_size)
         self.balls = cell_balls(tree, pts)
         self.m2l_src, self.m2l_dst, self.near_src, self.near_dst = mac_lists(tree, self.balls, theta)
         p_root = ctx.P if p_root is None else p_root
         p_leaf = p_root if p_leaf is None else p_leaf
         self.orders = cell_orders(tree, p_root, p_leaf)
-        self.off, self.pool_len, self.passes = fmm_passes(tree, self.balls, self.orders, self.m2l_src, self.m2l_dst)
+        self.off, self.pool_len, self.passes = fmm_passes(tree, self.balls, self.orders, self.m2l_src, self.m2l_dst,
+                                                          dim)
         self.ctx = ctx
         self.plans = [(op, N.Plan(ctx, op, it)) for (op, it, _, _, _) in self.passes if len(it)]
         leaves = np.nonzero(tree.leaf)[0]
@@ -186,14 +194,16 @@ class FMM:
         cells["first"] = tree.first[leaves]
         cells["count"] = tree.count[leaves]
         cells["order"] = self.orders[leaves]
+        cells["dim"] = dim
         self.leaves = leaves
         self.cellset = N.CellSet(ctx, cells, self.n)
         dev = f"cuda:{ctx.device}" if hasattr(ctx, "device") else "cuda"
         self.sorted_points = torch.from_numpy(pts[tree.order]).to(dev)
         self.M = torch.zeros(self.pool_len, dtype=torch.float64, device=dev)
         self.L = torch.zeros(self.pool_len, dtype=torch.float64, device=dev)
-        self.phi = torch.zeros(self.n, dtype=torch.float64, device=dev)
-        self.grad = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
+        shape = (self.n,) if dim == 1 else (self.n, dim)
+        self.phi = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.grad = torch.zeros(shape + (3,), dtype=torch.float64, device=dev)
         self.inverse = torch.from_numpy(np.argsort(tree.order)).to(dev)
 
     @property
@@ -245,8 +255,12 @@ class FMM:
         d = np.asarray(points[i, :3], dtype=np.float64) - self._srt[idx, :3]
         r = np.linalg.norm(d, axis=1)
         r[r == 0] = np.inf
-        q = self._srt[idx, 3]
-        return (q / r).sum(), -(q[:, None] * d / r[:, None] ** 3).sum(0)
+        q = self._srt[idx, 3:]  # (k, dim)
+        phi = (q / r[:, None]).sum(0)
+        grad = -(q[:, :, None] * (d / r[:, None] ** 3)[:, None, :]).sum(0)
+        if self.dim == 1:
+            return phi[0], grad[0]
+        return phi, grad
 
     def near_pairs(self):
         """Directed near-field leaf pairs (source, target), self pairs included."""

@@ -56,7 +56,7 @@ def test_cell_layout_matches_header():
     class Cell(ctypes.Structure):
         _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double),
                     ("offset", ctypes.c_uint64), ("first", ctypes.c_uint32), ("count", ctypes.c_uint32),
-                    ("order", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+                    ("order", ctypes.c_uint32), ("dim", ctypes.c_uint32)]
     assert ctypes.sizeof(Cell) == 48 == CELL_DTYPE.itemsize
     for name, (dt, off) in CELL_DTYPE.fields.items():
         assert getattr(Cell, name).offset == off

@@ -96,3 +96,33 @@ def test_fmm_mixed_orders_and_repeat():
     b = fmm.evaluate()
     torch.cuda.synchronize()
     assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+
+
+def test_fmm_vector_charges():
+    """Three charges per point (a vortex method's vector potential): every
+    component of the potential and of its gradient matches direct sums."""
+    import solidfmm_amd as sf
+    from solidfmm_amd.fmm import FMM
+    n = 12000
+    rng = np.random.default_rng(5)
+    pts = np.c_[O.plummer(n, 0x5011DF12), rng.uniform(-1, 1, (n, 3)) / n]
+    ctx = sf.Context(14)
+    fmm = FMM(ctx, pts, leaf_size=32, theta=0.5)
+    assert fmm.dim == 3
+    phi, grad = fmm.evaluate()
+    torch.cuda.synchronize()
+    phi = phi.cpu().numpy()
+    grad = grad.cpu().numpy()
+    assert phi.shape == (n, 3) and grad.shape == (n, 3, 3)
+    ep, eg = [], []
+    for i in rng.choice(n, 60, replace=False):
+        d = pts[i, :3] - pts[:, :3]
+        r = np.linalg.norm(d, axis=1)
+        r[i] = np.inf
+        want_p = (pts[:, 3:] / r[:, None]).sum(0)
+        want_g = -(pts[:, 3:, None] * (d / r[:, None] ** 3)[:, None, :]).sum(0)
+        near_p, near_g = fmm.near_field_at(pts, i)
+        ep.append(np.abs(phi[i] + near_p - want_p).max() / np.abs(want_p).max())
+        eg.append(np.abs(grad[i] + near_g - want_g).max() / np.abs(want_g).max())
+    print(f"vector FMM: potential {max(ep):.2e} gradient {max(eg):.2e}", flush=True)
+    assert max(ep) < 5e-5 and max(eg) < 5e-3

@@ -210,3 +210,67 @@ def test_cellset_validation(sfmm):
         cs.p2m(dev(np.zeros((4, 4))), torch.zeros(71, dtype=torch.float64, device="cuda"))
     with pytest.raises(sfmm.LogicError):
         cs.p2m(dev(np.zeros((3, 4))), torch.zeros(72, dtype=torch.float64, device="cuda"))
+
+
+@pytest.mark.parametrize("D", [2, 3, 4])
+def test_vector_valued_p2m_l2p(sfmm, D):
+    """D charges per point and D blocks per cell (solid<real>(P, D)): every
+    component equals the scalar operators applied to that component."""
+    orc = oracle.Oracle()
+    P = 9
+    rng = np.random.default_rng(300 + D)
+    counts = [0, 1, 33, 64, 90]
+    n = sum(counts)
+    cells = np.zeros(len(counts), dtype=sfmm.CELL_DTYPE)
+    pts = np.zeros((n, 3 + D))
+    first = off = 0
+    for c, k in enumerate(counts):
+        ctr = rng.uniform(-2, 2, 3)
+        cells[c] = (ctr[0], ctr[1], ctr[2], off, first, k, P, D)
+        pts[first:first + k, :3] = ctr + rng.uniform(-0.5, 0.5, (k, 3))
+        pts[first:first + k, 3:] = rng.uniform(-1, 1, (k, D))
+        first += k
+        off += D * P * (P + 1) + 2
+    pool0 = rng.standard_normal(off)
+    ctx = sfmm.Context(P)
+    cs = sfmm.CellSet(ctx, cells, n)
+    pool = dev(pool0)
+    cs.p2m(dev(pts), pool)
+    phi = torch.zeros(n * D, dtype=torch.float64, device="cuda")
+    grad = torch.zeros(n * D * 3, dtype=torch.float64, device="cuda")
+    cs.l2p(dev(pts), pool, phi, grad)
+    torch.cuda.synchronize()
+    got_pool = pool.cpu().numpy()
+    got_phi = phi.cpu().numpy().reshape(n, D)
+    got_grad = grad.cpu().numpy().reshape(n, D, 3)
+    blk = P * (P + 1)
+    for d in range(D):
+        # the scalar oracle on component d, with the cells' d-th blocks
+        sc = cells.copy()
+        sc["offset"] = cells["offset"] + d * blk
+        want_pool = p2m_oracle(orc, sc, np.c_[pts[:, :3], pts[:, 3 + d]], pool0.copy())
+        for c in sc:
+            o = int(c["offset"])
+            assert rel(got_pool[o:o + blk], want_pool[o:o + blk]) < 1e-13
+        phi_w, grad_w = l2p_oracle(orc, sc, np.c_[pts[:, :3], pts[:, 3 + d]], want_pool, np.zeros(n),
+                                   np.zeros((n, 3)))
+        assert rel(got_phi[:, d], phi_w) < 1e-12
+        assert rel(got_grad[:, d], grad_w) < 1e-12
+    # the gaps between cells are untouched
+    mask = np.ones(off, bool)
+    for c in cells:
+        mask[int(c["offset"]):int(c["offset"]) + D * blk] = False
+    assert np.array_equal(got_pool[mask], pool0[mask])
+
+
+def test_cellset_rejects_mixed_dims(sfmm):
+    ctx = sfmm.Context(6)
+    c = np.zeros(2, dtype=sfmm.CELL_DTYPE)
+    c[0] = (0, 0, 0, 0, 0, 1, 6, 3)
+    c[1] = (0, 0, 0, 200, 1, 1, 6, 2)
+    with pytest.raises(sfmm.LogicError):
+        sfmm.CellSet(ctx, c, 2)
+    c[1]["dim"] = 5
+    c[0]["dim"] = 5
+    with pytest.raises(sfmm.LogicError):
+        sfmm.CellSet(ctx, c, 2)
