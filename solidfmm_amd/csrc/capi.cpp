@@ -282,37 +282,63 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
     std::vector<sfmm_item> ordered;
     ordered.reserve(m);
     std::vector<uint64_t> scr_off;  // scratch offset per list position
+    std::vector<uint8_t> is_head;   // per list position: the item's block enters the reduction
     size_t scratch_reals = 0;
-    if (plan.scratch) scr_off.resize(m);
+    if (plan.scratch) {
+        scr_off.resize(m);
+        is_head.assign(m, 1);
+    }
+    // FP64 lane kernels in scratch mode share one scratch block among the
+    // items of a run: consecutive items of a launch, within one 64-item
+    // wave, with the same target and orders and a nonzero shift (the kernel
+    // adds a run's results across lanes, find_segments / segment_sum).
+    const bool runs = plan.scratch && ctx->precision == SFMM_FP64;
+    auto zero = [](const sfmm_item &it) { return it.x == 0.0 && it.y == 0.0 && it.z == 0.0; };
+    const sfmm_item *prev = nullptr;
+    uint64_t run_off = 0;
     for (size_t k = 0; k < m; ++k) {
         const Key &key = keys[k];
         if (k == 0 || keys[k - 1].round != key.round || keys[k - 1].family != key.family ||
-            keys[k - 1].pb != key.pb)
+            keys[k - 1].pb != key.pb) {
             plan.launches.push_back({{key.family, key.pb, false, ctx->precision, !plan.scratch}, ordered.size(), 0});
+            plan.launches.back().k.seg = runs && key.family == sfmm::kLaneKernels;
+            prev = nullptr;
+        }
         sfmm_plan::Launch &L = plan.launches.back();
         if (key.family == sfmm::kGenericKernel) {
             L.k.pk = std::max({L.k.pk, key.p_in, key.p_out});
         } else if (key.p_in != key.pb || key.p_out != key.pb) {
             L.k.rt = true;
         }
-        sfmm_item it = items[key.index];
-        if (plan.scratch) {  // the item's own block in the scratch pool
-            scr_off[list_pos[key.index]] = scratch_reals;
-            it.dst = scratch_reals;
-            scratch_reals += size_t(it.p_out) * (it.p_out + 1);
+        const sfmm_item &src_it = items[key.index];
+        sfmm_item it = src_it;
+        if (plan.scratch) {
+            const bool join = L.k.seg && prev && L.count % 64 != 0 && prev->dst == src_it.dst &&
+                              prev->p_in == src_it.p_in && prev->p_out == src_it.p_out && !zero(*prev) &&
+                              !zero(src_it);
+            if (!join) {  // a block of its own in the scratch pool
+                run_off = scratch_reals;
+                scratch_reals += size_t(it.p_out) * (it.p_out + 1);
+            } else {
+                is_head[list_pos[key.index]] = 0;
+            }
+            scr_off[list_pos[key.index]] = run_off;
+            it.dst = run_off;
         }
+        prev = &src_it;
         ordered.push_back(it);
         L.count++;
     }
     // Reduction tables: targets in order of first appearance, each with its
-    // items in list order.
+    // scratch blocks in the list order of their first items (a run's block
+    // holds the sum of its items, added in the kernel).
     std::vector<sfmm::RedTarget> rt;
     std::vector<sfmm::RedItem> ri;
     if (plan.scratch) {
         FlatMap tid;
         std::vector<uint32_t> count;
         std::vector<uint32_t> of_pos(m);  // target id per list position
-        size_t pos = 0;
+        size_t pos = 0, heads = 0;
         for (size_t i = 0; i < n; ++i) {
             const sfmm_item &it = items[i];
             if (it.p_in == 0 || it.p_out == 0) continue;
@@ -322,8 +348,12 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
                 rt.push_back({it.dst, 0, 0, 0, 0});
                 count.push_back(0);
             }
-            of_pos[pos++] = t;
-            count[t]++;
+            of_pos[pos] = t;
+            if (is_head[pos]) {
+                count[t]++;
+                ++heads;
+            }
+            ++pos;
             rt[t].len = std::max(rt[t].len, written_reals(op, it));
         }
         uint32_t first = 0;
@@ -332,13 +362,15 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
             first += count[t];
             count[t] = 0;
         }
-        ri.resize(m);
+        ri.resize(heads);
         pos = 0;
         for (size_t i = 0; i < n; ++i) {
             const sfmm_item &it = items[i];
             if (it.p_in == 0 || it.p_out == 0) continue;
-            const uint32_t t = of_pos[pos];
-            ri[rt[t].first + count[t]++] = {scr_off[pos], written_reals(op, it), 0};
+            if (is_head[pos]) {
+                const uint32_t t = of_pos[pos];
+                ri[rt[t].first + count[t]++] = {scr_off[pos], written_reals(op, it), 0};
+            }
             ++pos;
         }
         for (size_t t = 0; t < rt.size(); ++t) rt[t].count = count[t];

@@ -227,9 +227,60 @@ __device__ __forceinline__ void zero_shift_add(const double *__restrict__ S, dou
     for (uint32_t i = 0; i < len; ++i) D[i] = (acc ? D[i] : 0.0) + S[i];
 }
 
-// One lane, one translation (see the head of this file for RT).  acc:
-// target += T(source); !acc: target = T(source) over the degrees the
-// translation defines (scratch blocks of a segmented reduction).
+// Segmented scratch mode: lanes whose items share a scratch block (a run of
+// one target's items with equal orders, laid out by the plan) add their
+// results by a fixed tree of shuffles and the run's first lane stores the
+// sum, so a run of k items writes one block instead of k.
+struct Segments {
+    uint32_t same = 0;  // bit j: lane + 2^j belongs to this lane's run
+    bool head = true;   // this lane stores its run's sum
+};
+
+__device__ __forceinline__ double shfl_down(double v, int d)
+{
+    return __shfl_down(v, static_cast<unsigned>(d), 64);
+}
+
+// Runs are identified by equal keys (the item's block offset; lanes without
+// an item get a key of their own).  Call with all lanes of the wave active.
+__device__ __forceinline__ Segments find_segments(uint64_t key)
+{
+    Segments g;
+    const int lane = static_cast<int>(threadIdx.x & 63);
+    const uint32_t lo = static_cast<uint32_t>(key), hi = static_cast<uint32_t>(key >> 32);
+    for (int j = 0; j < 6; ++j) {
+        const int d = 1 << j;
+        const uint32_t olo = __shfl_down(lo, static_cast<unsigned>(d), 64);
+        const uint32_t ohi = __shfl_down(hi, static_cast<unsigned>(d), 64);
+        if (lane + d < 64 && olo == lo && ohi == hi) g.same |= 1u << j;
+    }
+    const uint32_t plo = __shfl_up(lo, 1u, 64), phi = __shfl_up(hi, 1u, 64);
+    g.head = lane == 0 || plo != lo || phi != hi;
+    return g;
+}
+
+// v of the run's first lane becomes the sum over the run: after the step
+// with offset d, lane l holds the sum over lanes l .. l + 2d - 1 of its run.
+template <int N>
+__device__ __forceinline__ void segment_sum(const Segments &g, double (&re)[N + 1], double (&im)[N + 1])
+{
+    static_for<0, 6>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const bool take = (g.same >> j) & 1u;
+        static_for<0, N + 1>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const double a = shfl_down(re[m], 1 << j);
+            const double b = shfl_down(im[m], 1 << j);
+            re[m] += take ? a : 0.0;
+            im[m] += take ? b : 0.0;
+        });
+    });
+}
+
+// One lane, one translation (see the head of this file for RT).  acc = 1:
+// target += T(source); acc = 0: target = T(source) over the degrees the
+// translation defines (scratch blocks of a segmented reduction); acc = 2:
+// as 0, with the lanes of a run adding into one block (Segments).
 template <int OP, int PI, int PO, bool RT>
 __global__ void __launch_bounds__(64)
     k_translate(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
@@ -239,6 +290,12 @@ __global__ void __launch_bounds__(64)
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int PE = PI < PO ? PI : PO;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    Segments seg;
+    if (acc == 2) {  // before any lane leaves
+        uint64_t key = ~uint64_t(threadIdx.x & 63);
+        if (t < n_items) key = items[t].dst;
+        seg = find_segments(key);
+    }
     if (t >= n_items) return;
 
     const sfmm_item it = items[t];
@@ -253,7 +310,7 @@ __global__ void __launch_bounds__(64)
 
     if constexpr (OP != SFMM_M2L) {
         if (x * x + y * y + z * z == 0.0) {
-            zero_shift_add(S, D, RT ? pend : uint32_t(PE), acc != 0);
+            zero_shift_add(S, D, RT ? pend : uint32_t(PE), acc == 1);
             return;
         }
     }
@@ -267,7 +324,17 @@ __global__ void __launch_bounds__(64)
     const uint32_t pout_w = RT ? wave_max7(OP == SFMM_L2L ? pend : pout) : uint32_t(PO);
     auto out_on = [&](int N) { return !RT || uint32_t(N) < (OP == SFMM_L2L ? pend : pout); };
     // where degree N's old target values come from: the target, or zeros
-    const double *__restrict__ Dr = acc ? D : kZeroBlock;
+    const double *__restrict__ Dr = acc == 1 ? D : kZeroBlock;
+    // store degree N: in segmented mode the run's sum, by its first lane
+    auto finish_degree = [&](auto Nc, const double2(&cur)[PO], double(&re)[decltype(Nc)::value + 1],
+                             double(&im)[decltype(Nc)::value + 1], bool on) {
+        constexpr int N = decltype(Nc)::value;
+        if (acc == 2) {
+            segment_sum<N>(seg, re, im);
+            on = on && seg.head;
+        }
+        add_store_degree<N>(D, cur, re, im, on);
+    };
     auto read_from = [&](int N) { return out_on(N) ? Dr : kZeroBlock; };
 
     // Euler data (microkernel_generic.cpp:61-74).
@@ -388,7 +455,7 @@ __global__ void __launch_bounds__(64)
                     im[m] = ai;
                 });
                 backward<N, true>(R, rp[N], re, im);
-                add_store_degree<N>(D, cur, re, im, out_on(N));
+                finish_degree(ic<N>{}, cur, re, im, out_on(N));
             }
         });
     } else if constexpr (OP == SFMM_M2M) {
@@ -444,7 +511,7 @@ __global__ void __launch_bounds__(64)
                         im[m] = ai;
                     });
                     backward<N, false>(R, sb, re, im);
-                    add_store_degree<N>(D, cur, re, im, out_on(N));
+                    finish_degree(ic<N>{}, cur, re, im, out_on(N));
                 }
                 sb *= r;
             }
@@ -505,7 +572,7 @@ __global__ void __launch_bounds__(64)
                         im[m] = ai;
                     });
                     backward<N, true>(R, bs[N], re, im);
-                    add_store_degree<N>(D, cur, re, im, out_on(N));
+                    finish_degree(ic<N>{}, cur, re, im, out_on(N));
                 }
             }
         });

@@ -109,7 +109,8 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
         lane::kernel_fn f = t.f[op][k.pk - 1];
         const uint32_t block = 64;
         const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
-        hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n), acc);
+        hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n),
+                           (!k.acc && k.seg) ? 2 : acc);
         return hipGetLastError();
     }
     case kMfmaKernels: {

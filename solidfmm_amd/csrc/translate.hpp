@@ -72,6 +72,9 @@ struct KernelChoice {
     bool rt;
     int precision = SFMM_FP64;
     bool acc = true;
+    // scratch mode with runs of one target's items sharing a block (FP64
+    // lane kernels only; the plan lays the runs out)
+    bool seg = false;
 };
 
 // Segmented reduction: target block t += sum of its items' scratch blocks,
