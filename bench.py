@@ -406,6 +406,25 @@ def fmm_mode(args):
         ep.append(abs(phi[i] + near_p - want_p) / abs(want_p))
         eg.append(np.linalg.norm(grad[i] + near_g - want_g) / np.linalg.norm(want_g))
     n_tr = fmm.translations
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the reference's translations of one evaluation (M2M, M2L and L2L
+        # passes in order) on the host cores; random pools of the same size
+        import oracle
+        if oracle.Reference.available():
+            ref = oracle.Reference()
+            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+            mpool = rng.uniform(-1, 1, fmm.pool_len)
+            lpool = np.zeros(fmm.pool_len)
+            secs = 0.0
+            for op, it, _, _, _ in fmm.passes:
+                src = lpool if op == 2 else mpool
+                dst = mpool if op == 0 else lpool
+                secs += ref.translate(op, 20, it, src, dst, threads=threads, repeats=1, chunk=0)
+            cpu = {"value": n_tr / secs, "unit": "translations/s", "cores": threads, "kind": "reference",
+                   "sample": f"vorticle/solidfmm 1.5 single-thread *_unchecked on {threads} std::threads, targets "
+                             "hash-partitioned, the M2M/M2L/L2L passes of one evaluation in order (P2M/L2P not "
+                             f"timed); {secs:.2f} s"}
     line = {"metric": "far-field FMM evaluations of a Plummer sphere: translations/s over P2M..L2P",
             "value": n_tr / (ms * 1e-3), "unit": "translations/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -416,7 +435,7 @@ def fmm_mode(args):
                                    "6 (leaves); one step = P2M + M2M + M2L + L2L + L2P (near field not included)",
                        "points": args.points, "cells": int(len(fmm.tree)), "translations": n_tr,
                        "m2l": int(len(fmm.m2l_src)), "near_pairs": int(len(fmm.near_src)), "setup_s": setup_s},
-            "points_per_s": args.points / (ms * 1e-3), "phase_ms": phases,
+            "points_per_s": args.points / (ms * 1e-3), "phase_ms": phases, "cpu_baseline": cpu,
             "rel_err_vs_direct": {"potential_max": float(np.max(ep)), "potential_p50": float(np.median(ep)),
                                   "gradient_max": float(np.max(eg)), "gradient_p50": float(np.median(eg))}}
     print(json.dumps(line), flush=True)
