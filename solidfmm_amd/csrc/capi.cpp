@@ -288,10 +288,11 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         scr_off.resize(m);
         is_head.assign(m, 1);
     }
-    // FP64 lane kernels in scratch mode share one scratch block among the
-    // items of a run: consecutive items of a launch, within one 64-item
-    // wave, with the same target and orders and a nonzero shift (the kernel
-    // adds a run's results across lanes, find_segments / segment_sum).
+    // FP64 lane and MFMA kernels in scratch mode share one scratch block
+    // among the items of a run: consecutive items of a launch, within one
+    // 64-item wave (lane kernels) or 16-item group (MFMA kernels), with the
+    // same target and orders and a nonzero shift (the kernels add a run's
+    // results across lanes).
     const bool runs = plan.scratch && ctx->precision == SFMM_FP64;
     auto zero = [](const sfmm_item &it) { return it.x == 0.0 && it.y == 0.0 && it.z == 0.0; };
     const sfmm_item *prev = nullptr;
@@ -301,7 +302,8 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         if (k == 0 || keys[k - 1].round != key.round || keys[k - 1].family != key.family ||
             keys[k - 1].pb != key.pb) {
             plan.launches.push_back({{key.family, key.pb, false, ctx->precision, !plan.scratch}, ordered.size(), 0});
-            plan.launches.back().k.seg = runs && key.family == sfmm::kLaneKernels;
+            plan.launches.back().k.seg =
+                runs && (key.family == sfmm::kLaneKernels || key.family == sfmm::kMfmaKernels);
             prev = nullptr;
         }
         sfmm_plan::Launch &L = plan.launches.back();
@@ -313,7 +315,8 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         const sfmm_item &src_it = items[key.index];
         sfmm_item it = src_it;
         if (plan.scratch) {
-            const bool join = L.k.seg && prev && L.count % 64 != 0 && prev->dst == src_it.dst &&
+            const size_t wave = key.family == sfmm::kLaneKernels ? 64 : 16;
+            const bool join = L.k.seg && prev && L.count % wave != 0 && prev->dst == src_it.dst &&
                               prev->p_in == src_it.p_in && prev->p_out == src_it.p_out && !zero(*prev) &&
                               !zero(src_it);
             if (!join) {  // a block of its own in the scratch pool

@@ -116,8 +116,9 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
     case kMfmaKernels: {
         const double *table = mfma::device_table();
         if (!table) return hipErrorOutOfMemory;
-        return k.rt ? launch_mfma<true>(op, k.pk, items, n, src, dst, table, stream, acc)
-                    : launch_mfma<false>(op, k.pk, items, n, src, dst, table, stream, acc);
+        const int mode = (!k.acc && k.seg) ? 2 : acc;
+        return k.rt ? launch_mfma<true>(op, k.pk, items, n, src, dst, table, stream, mode)
+                    : launch_mfma<false>(op, k.pk, items, n, src, dst, table, stream, mode);
     }
     default: return launch_generic(op, k.pk, items, n, src, dst, stream, acc);
     }

@@ -204,8 +204,12 @@ __device__ __forceinline__ uint32_t wave_max7(uint32_t v)
 template <int OP, int PI, int PO, bool RT>
 __device__ __forceinline__ void translate_group(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                                                 const sfmm_item *__restrict__ items, uint32_t n_items, uint32_t g,
-                                                const double *tab, int lane, bool acc)
+                                                const double *tab, int lane, int mode)
 {
+    // mode 1: target += T(source); 0: scratch block = T(source); 2: as 0,
+    // with the items of a run (consecutive translations of the group with
+    // one scratch block) added across lanes and stored by the run's first
+    const bool acc = mode == 1;
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int PE = PI < PO ? PI : PO;
     constexpr int NSL = 4 * ((PM + 15) / 16);  // slots incl. padding to whole row blocks
@@ -217,6 +221,21 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
     // Every lane stays active to the end: the MFMAs need the whole wave.
     sfmm_item it{};
     if (valid) it = items[idx];
+    // runs within the group: bit j of `same` says translation t + 2^j is in
+    // t's run; the run's first translation stores the sum
+    uint32_t same = 0;
+    bool head = true;
+    if (mode == 2) {
+        const uint64_t key = valid ? it.dst : ~uint64_t(lane);
+        const uint32_t lo = uint32_t(key), hi = uint32_t(key >> 32);
+        static_for<0, 4>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const uint32_t olo = __shfl_down(lo, 1u << j, 64), ohi = __shfl_down(hi, 1u << j, 64);
+            if (t + (1 << j) < 16 && olo == lo && ohi == hi) same |= 1u << j;
+        });
+        const uint32_t plo = __shfl_up(lo, 1u, 64), phi = __shfl_up(hi, 1u, 64);
+        head = t == 0 || plo != lo || phi != hi;
+    }
     const double *__restrict__ S = src_pool + it.src;
     double *__restrict__ D = dst_pool + it.dst;
     double x = it.x, y = it.y, z = it.z;
@@ -344,10 +363,26 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             }
         });
     };
-    auto store_sum = [&](auto Nc, const double(&orr)[NSL], const double(&oi)[NSL], const double(&xr)[NSL],
-                         const double(&xi)[NSL]) {
+    auto store_sum = [&](auto Nc, const double(&orr)[NSL], const double(&oi)[NSL], double(&xr)[NSL],
+                         double(&xi)[NSL]) {
         constexpr int N = decltype(Nc)::value;
-        const bool on = out_on(N);
+        bool on = out_on(N);
+        if (mode == 2) {  // fixed tree over the run's translations (lanes t + 2^j, same q)
+            static_for<0, 4>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                const bool take = (same >> j) & 1u;
+                static_for<0, NSL>([&](auto sc) {
+                    constexpr int s = decltype(sc)::value;
+                    if constexpr (4 * s <= N) {
+                        const double a = __shfl_down(xr[s], 1u << j, 64);
+                        const double b = __shfl_down(xi[s], 1u << j, 64);
+                        xr[s] += take ? a : 0.0;
+                        xi[s] += take ? b : 0.0;
+                    }
+                });
+            });
+            on = on && head;
+        }
         static_for<0, NSL>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
             if constexpr (4 * s <= N) {
@@ -573,7 +608,7 @@ __global__ void __launch_bounds__(64 * kWaves)
         // each group reads its operands from LDS where they are used.
         int base = 0;
         asm volatile("" : "+s"(base));
-        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab + base, lane, acc != 0);
+        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab + base, lane, acc);
     }
 }
 
