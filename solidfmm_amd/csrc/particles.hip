@@ -32,6 +32,21 @@ namespace {
 constexpr int kP2MThreads = 64;
 constexpr int kL2PThreads = 64;
 
+// 1/k for the recurrences' divisors (2j on the diagonal, (n+m)(n-m) =
+// (n+m) times (n-m) down a column), so the inner loops multiply instead of
+// divide.
+constexpr int kInvSize = 2 * 88;  // n + m < 2 * 86 for every order the library takes
+struct InvTable {
+    double v[kInvSize];
+};
+constexpr InvTable make_inv()
+{
+    InvTable t{};
+    for (int k = 1; k < kInvSize; ++k) t.v[k] = 1.0 / k;
+    return t;
+}
+__constant__ constexpr InvTable kInv = make_inv();
+
 template <int D>
 __global__ void __launch_bounds__(kP2MThreads)
     k_p2m(const double *__restrict__ pts, const sfmm_cell *__restrict__ cells, uint32_t n_cells,
@@ -60,7 +75,7 @@ __global__ void __launch_bounds__(kP2MThreads)
                     di = y / 2;
                 }
                 for (int j = 2; j <= m; ++j) {
-                    const double f = 1.0 / (2.0 * j);
+                    const double f = kInv.v[2 * j];
                     const double a = dr, b = di;
                     dr = (a * x - b * y) * f;
                     di = (a * y + b * x) * f;
@@ -72,7 +87,7 @@ __global__ void __launch_bounds__(kP2MThreads)
                     ai = z * ai;
                     const double r2 = x * x + y * y + z * z;
                     for (int nn = m + 2; nn <= n; ++nn) {
-                        const double f = 1.0 / (double(nn + m) * double(nn - m));
+                        const double f = kInv.v[nn + m] * kInv.v[nn - m];
                         const double tr = (ar * z * (2 * nn - 1) - r2 * br) * f;
                         const double ti = (ai * z * (2 * nn - 1) - r2 * bi) * f;
                         br = ar;
@@ -123,7 +138,7 @@ __global__ void __launch_bounds__(kL2PThreads)
                     gdr[0] = 0.5;
                     gdi[1] = 0.5;
                 } else if (m >= 2) {
-                    const double f = 1.0 / (2.0 * m);
+                    const double f = kInv.v[2 * m];
                     const double a = dr, b = di;
                     const double ax = gdr[0], ay = gdr[1], az = gdr[2], bx = gdi[0], by = gdi[1], bz = gdi[2];
                     dr = (a * x - b * y) * f;
@@ -157,7 +172,7 @@ __global__ void __launch_bounds__(kL2PThreads)
                         gar[2] += br;
                         gai[2] += bi;
                     } else if (n >= m + 2) {
-                        const double f = 1.0 / (double(n + m) * double(n - m));
+                        const double f = kInv.v[n + m] * kInv.v[n - m];
                         const double k2 = 2.0 * n - 1.0;
                         const double tr = (ar * z * k2 - r2 * br) * f;
                         const double ti = (ai * z * k2 - r2 * bi) * f;
