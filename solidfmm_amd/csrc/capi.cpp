@@ -814,7 +814,7 @@ struct sfmm_staging {
     size_t h_src_cap = 0, h_dst_cap = 0;        // reals
     double *d_src = nullptr, *d_dst = nullptr;
     size_t d_src_cap = 0, d_dst_cap = 0;  // reals
-    void *d_plan = nullptr;
+    unsigned char *d_plan = nullptr;
     size_t d_plan_cap = 0;  // bytes
     hipStream_t stream = nullptr;
 };
@@ -958,7 +958,7 @@ int sfmm_staging_translate(sfmm_staging *st, int op, const sfmm_item *items, siz
     int rc = sfmm_plan_prepare(ctx, op, items, n, &plan);
     if (rc == SFMM_OK && plan->n_items) {
         const size_t bytes = sfmm_plan_device_bytes(plan);
-        e = grow_dev(reinterpret_cast<unsigned char *&>(st->d_plan), st->d_plan_cap, bytes, 1);
+        e = grow_dev(st->d_plan, st->d_plan_cap, bytes, 1);
         if (e != hipSuccess) {
             std::lock_guard<std::mutex> lock(ctx->mu);
             rc = fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
