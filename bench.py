@@ -369,7 +369,7 @@ def fmm_mode(args):
     pts = np.c_[O.plummer(args.points, seed), rng.uniform(0.5, 1.5, args.points) / args.points]
     t0 = time.perf_counter()
     ctx = solidfmm_amd.Context(20)
-    fmm = FMM(ctx, pts, leaf_size=64, theta=args.theta, p_root=20, p_leaf=6)
+    fmm = FMM(ctx, pts, leaf_size=64, theta=args.theta, p_root=20, p_leaf=6, autotune=not args.no_autotune)
     setup_s = time.perf_counter() - t0
     for _ in range(args.warmup):
         fmm.evaluate()

@@ -172,7 +172,7 @@ class FMM:
     (n, dim, 3), holding the contributions of every source outside the
     point's near pairs."""
 
-    def __init__(self, ctx, points, leaf_size=64, theta=0.5, p_root=None, p_leaf=None):
+    def __init__(self, ctx, points, leaf_size=64, theta=0.5, p_root=None, p_leaf=None, autotune=False):
         import torch
         pts = np.ascontiguousarray(points, dtype=np.float64)
         self.n = len(pts)
@@ -186,6 +186,11 @@ class FMM:
         self.off, self.pool_len, self.passes = fmm_passes(tree, self.balls, self.orders, self.m2l_src, self.m2l_dst,
                                                           dim)
         self.ctx = ctx
+        if autotune:  # pick the kernel family per order bucket on this tree's own lists
+            for op in (M2M, M2L, L2L):
+                its = [it for (o, it, _, _, _) in self.passes if o == op and len(it)]
+                if its:
+                    ctx.autotune(op, np.concatenate(its))
         self.plans = [(op, N.Plan(ctx, op, it)) for (op, it, _, _, _) in self.passes if len(it)]
         leaves = np.nonzero(tree.leaf)[0]
         cells = np.zeros(len(leaves), dtype=N.CELL_DTYPE)
