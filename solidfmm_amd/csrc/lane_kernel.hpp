@@ -290,11 +290,16 @@ __global__ void __launch_bounds__(64)
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int PE = PI < PO ? PI : PO;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    // runs (acc == 2) exist in the per-item-order kernels only; the plan
+    // sends scratch launches with runs there, so the fixed-order kernels
+    // carry no code for them
     Segments seg;
-    if (acc == 2) {  // before any lane leaves
-        uint64_t key = ~uint64_t(threadIdx.x & 63);
-        if (t < n_items) key = items[t].dst;
-        seg = find_segments(key);
+    if constexpr (RT) {
+        if (acc == 2) {  // before any lane leaves
+            uint64_t key = ~uint64_t(threadIdx.x & 63);
+            if (t < n_items) key = items[t].dst;
+            seg = find_segments(key);
+        }
     }
     if (t >= n_items) return;
 
@@ -329,9 +334,11 @@ __global__ void __launch_bounds__(64)
     auto finish_degree = [&](auto Nc, const double2(&cur)[PO], double(&re)[decltype(Nc)::value + 1],
                              double(&im)[decltype(Nc)::value + 1], bool on) {
         constexpr int N = decltype(Nc)::value;
-        if (acc == 2) {
-            segment_sum<N>(seg, re, im);
-            on = on && seg.head;
+        if constexpr (RT) {
+            if (acc == 2) {
+                segment_sum<N>(seg, re, im);
+                on = on && seg.head;
+            }
         }
         add_store_degree<N>(D, cur, re, im, on);
     };

@@ -105,12 +105,15 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
     if (n > 0xFFFFFFFFull) return hipErrorInvalidValue;
     switch (k.family) {
     case kLaneKernels: {
-        const lane::Table &t = k.rt ? lane::rt_table() : lane::fixed_table();
+        // scratch launches with runs use the per-item-order kernels (the
+        // fixed-order ones have no code for runs)
+        const bool runs = !k.acc && k.seg;
+        const lane::Table &t = (k.rt || runs) ? lane::rt_table() : lane::fixed_table();
         lane::kernel_fn f = t.f[op][k.pk - 1];
         const uint32_t block = 64;
         const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
         hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n),
-                           (!k.acc && k.seg) ? 2 : acc);
+                           runs ? 2 : acc);
         return hipGetLastError();
     }
     case kMfmaKernels: {
