@@ -139,6 +139,15 @@ int sfmm_plan_create(sfmm_context *ctx, int op, const sfmm_item *host_items, siz
 int sfmm_plan_prepare(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, sfmm_plan **plan);
 size_t sfmm_plan_device_bytes(const sfmm_plan *plan);
 int sfmm_plan_bind(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *stream);
+/* As sfmm_plan_bind, but the tables go through caller-owned pinned host
+ * memory `host_mem` (at least sfmm_plan_host_bytes bytes): they are copied
+ * there on the host and from there to the device asynchronously, so the
+ * call returns at once (a copy from the plan's own pageable memory makes
+ * the HIP runtime stage it synchronously).  host_mem must stay unchanged
+ * until the stream has passed the copy. */
+size_t sfmm_plan_host_bytes(const sfmm_plan *plan);
+int sfmm_plan_bind_staged(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *host_mem, size_t host_bytes,
+                          void *stream);
 void sfmm_plan_destroy(sfmm_plan *plan);
 size_t sfmm_plan_launches(const sfmm_plan *plan);
 

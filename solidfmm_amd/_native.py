@@ -51,6 +51,8 @@ SIGNATURES = {
     "sfmm_plan_prepare": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _c.POINTER(_vp)]),
     "sfmm_plan_bind": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp]),
     "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
+    "sfmm_plan_host_bytes": (_c.c_size_t, [_vp]),
+    "sfmm_plan_bind_staged": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
     "sfmm_device": (_c.c_int, [_vp]),
     "sfmm_server_create": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _c.c_size_t, _c.c_int, _c.c_size_t,
                                       _c.c_int, _c.POINTER(_vp)]),

@@ -555,6 +555,30 @@ int sfmm_plan_prepare(sfmm_context *ctx, int op, const sfmm_item *items, size_t 
 
 size_t sfmm_plan_device_bytes(const sfmm_plan *plan) { return plan ? plan->dev_bytes : 0; }
 
+size_t sfmm_plan_host_bytes(const sfmm_plan *plan) { return plan ? plan->blob.size() : 0; }
+
+int sfmm_plan_bind_staged(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *host_mem, size_t host_bytes,
+                          void *stream)
+{
+    if (!plan) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = plan->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (plan->owns_mem) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan owns its device memory");
+    if (plan->dev_bytes && (!dev_mem || dev_bytes < plan->dev_bytes || (reinterpret_cast<uintptr_t>(dev_mem) & 255u)))
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: device buffer missing, misaligned or too small");
+    if (!plan->blob.empty() && (!host_mem || host_bytes < plan->blob.size()))
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: host staging missing or too small");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess && !plan->blob.empty()) {
+        std::memcpy(host_mem, plan->blob.data(), plan->blob.size());
+        e = hipMemcpyAsync(dev_mem, host_mem, plan->blob.size(), hipMemcpyHostToDevice,
+                           static_cast<hipStream_t>(stream));
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync");
+    plan->d_mem = static_cast<unsigned char *>(dev_mem);
+    return SFMM_OK;
+}
+
 int sfmm_plan_bind(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *stream)
 {
     if (!plan) return SFMM_ERR_LOGIC;
@@ -816,6 +840,8 @@ struct sfmm_staging {
     size_t d_src_cap = 0, d_dst_cap = 0;  // reals
     unsigned char *d_plan = nullptr;
     size_t d_plan_cap = 0;  // bytes
+    double *h_plan = nullptr;  // pinned: the plan's tables pass through it
+    size_t h_plan_cap = 0;     // reals
     hipStream_t stream = nullptr;
 };
 
@@ -901,6 +927,7 @@ void sfmm_staging_destroy(sfmm_staging *st)
     if (st->d_src) (void)hipFree(st->d_src);
     if (st->d_dst) (void)hipFree(st->d_dst);
     if (st->d_plan) (void)hipFree(st->d_plan);
+    if (st->h_plan) (void)hipHostFree(st->h_plan);
     if (st->stream) (void)hipStreamDestroy(st->stream);
     delete st;
 }
@@ -963,7 +990,14 @@ int sfmm_staging_translate(sfmm_staging *st, int op, const sfmm_item *items, siz
             std::lock_guard<std::mutex> lock(ctx->mu);
             rc = fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
         }
-        if (rc == SFMM_OK) rc = sfmm_plan_bind(plan, st->d_plan, st->d_plan_cap, st->stream);
+        const size_t hb = sfmm_plan_host_bytes(plan);
+        if (rc == SFMM_OK && grow_host(st->h_plan, st->h_plan_cap, (hb + 7) / 8) != hipSuccess) {
+            std::lock_guard<std::mutex> lock(ctx->mu);
+            rc = fail(ctx, SFMM_ERR_ALLOC, "hipHostMalloc: plan staging");
+        }
+        if (rc == SFMM_OK)
+            rc = sfmm_plan_bind_staged(plan, st->d_plan, st->d_plan_cap, st->h_plan, st->h_plan_cap * sizeof(double),
+                                       st->stream);
         if (rc == SFMM_OK) rc = sfmm_plan_execute(plan, st->d_src, src_len, st->d_dst, dst_len, st->stream);
         if (rc == SFMM_OK) {
             e = hipMemcpyAsync(st->h_dst, st->d_dst, dst_len * sizeof(double), hipMemcpyDeviceToHost, st->stream);

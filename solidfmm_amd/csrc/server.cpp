@@ -122,6 +122,7 @@ struct sfmm_server {
     std::vector<hipStream_t> streams;
     std::vector<int> free_slots;
     void *arena_mem = nullptr;
+    unsigned char *arena_host = nullptr;  // pinned mirror: plan tables pass through it
     std::unique_ptr<Arena> arena;
     std::vector<int32_t> readers[2], writers[2];
     hipEvent_t ref = nullptr;
@@ -200,7 +201,10 @@ struct sfmm_server {
         if (hipEventCreate(&r.ev_start) != hipSuccess || hipEventCreate(&r.ev_end) != hipSuccess) return false;
         if (hipEventRecord(r.ev_start, s) != hipSuccess) return false;
         void *mem = static_cast<unsigned char *>(arena_mem) + r.arena_off;
-        if (sfmm_plan_bind(r.plan, mem, r.arena_len, s) != SFMM_OK) return false;
+        // the request's range of the pinned mirror stays reserved with its
+        // device range until completion, so the copy reads stable memory
+        if (sfmm_plan_bind_staged(r.plan, mem, r.arena_len, arena_host + r.arena_off, r.arena_len, s) != SFMM_OK)
+            return false;
         if (sfmm_plan_execute(r.plan, pool[sp], pool_len[sp], pool[dp], pool_len[dp], s) != SFMM_OK) return false;
         return hipEventRecord(r.ev_end, s) == hipSuccess;
     }
@@ -353,12 +357,15 @@ int sfmm_server_create(sfmm_context *ctx, double *m_pool, size_t m_len, double *
         }
     }
     if (e == hipSuccess) e = hipMalloc(&sv->arena_mem, arena_bytes);
+    if (e == hipSuccess)
+        e = hipHostMalloc(reinterpret_cast<void **>(&sv->arena_host), arena_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&sv->ref);
     if (e == hipSuccess) e = hipEventRecord(sv->ref, nullptr);
     if (e == hipSuccess) e = hipEventSynchronize(sv->ref);
     if (e != hipSuccess) {
         for (hipStream_t st : sv->streams) (void)hipStreamDestroy(st);
         if (sv->arena_mem) (void)hipFree(sv->arena_mem);
+        if (sv->arena_host) (void)hipHostFree(sv->arena_host);
         if (sv->ref) (void)hipEventDestroy(sv->ref);
         return e == hipErrorOutOfMemory ? SFMM_ERR_ALLOC : SFMM_ERR_HIP;
     }
@@ -469,6 +476,7 @@ void sfmm_server_destroy(sfmm_server *sv)
         (void)hipStreamDestroy(st);
     }
     (void)hipFree(sv->arena_mem);
+    (void)hipHostFree(sv->arena_host);
     (void)hipEventDestroy(sv->ref);
     delete sv;
 }
