@@ -285,7 +285,8 @@ def serving_mode(args):
     root to 6 at the leaves) cut into requests of U[1k, 50k] translations.
     The kernel families are autotuned on the workload's own lists (FP64).
     One-at-a-time GPU time is measured first (each request alone,
-    synchronised); then a saturated run of the continuous-batching loop
+    synchronised); then, after one untimed pass, a saturated run of the
+    continuous-batching loop
     (--slots streams, --budget-mib device arena; every request available at
     once) gives the loop's capacity, and the requests arrive as a Poisson
     process at --load x that capacity.  Reports translations/s and p50/p99
@@ -329,6 +330,9 @@ def serving_mode(args):
                "alone_gpu_ms": sum(gpu_ms), "alone_translations_per_s": total / (sum(gpu_ms) * 1e-3)}
         cls = S.NativeServer if args.server == "native" else S.Server
         server = cls(ctx, pools, len(tree), slots=args.slots, budget_bytes=args.budget_mib << 20)
+        # one untimed pass warms the loop (first use of its arena, pinned
+        # pool, threads and the plan caches)
+        server.run(S.make_requests(passes, seed), realtime=False)
         reqs = S.make_requests(passes, seed)
         row["saturated"] = server.run(reqs, realtime=False)
         # the loop's own capacity: requests/s with every request available

@@ -148,6 +148,12 @@ int sfmm_plan_bind(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *strea
 size_t sfmm_plan_host_bytes(const sfmm_plan *plan);
 int sfmm_plan_bind_staged(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *host_mem, size_t host_bytes,
                           void *stream);
+/* The two halves of sfmm_plan_bind_staged, for a preparation thread and a
+ * launching thread: copy the tables into pinned host memory (host only),
+ * then bind from there (an asynchronous copy on `stream`; host_tables must
+ * stay unchanged until the stream has passed it). */
+int sfmm_plan_copy_tables(const sfmm_plan *plan, void *host_mem, size_t host_bytes);
+int sfmm_plan_bind_pinned(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, const void *host_tables, void *stream);
 void sfmm_plan_destroy(sfmm_plan *plan);
 size_t sfmm_plan_launches(const sfmm_plan *plan);
 
@@ -259,7 +265,9 @@ int sfmm_device(const sfmm_context *ctx);
  * runs requests on `slots` streams of its own:
  *   - sfmm_server_submit copies a request (operator, items, the cells it
  *     reads and writes) and queues it in arrival order; `workers` host
- *     threads prepare plans (sfmm_plan_prepare) while the GPU runs others;
+ *     threads prepare plans (sfmm_plan_prepare) while the GPU runs others,
+ *     at a lower CPU priority (niceness 5) than the server's scheduling
+ *     thread, which should have a core to itself;
  *   - the oldest waiting request is admitted when it is prepared, a slot is
  *     free, its device memory fits in the server's arena of `arena_bytes`,
  *     and it has no hazard with a request in flight (it writes no cell one
@@ -267,9 +275,10 @@ int sfmm_device(const sfmm_context *ctx);
  *     arrival order, so the pools end as if the requests ran one by one;
  *   - admitted requests are bound into the arena and launched between two
  *     events; a scheduler thread polls completions.
- * Times: sfmm_server_request_info gives arrival and admission (host clock)
- * and completion (GPU clock), in ms since the server's creation, and the
- * request's own GPU time; rc is its status (a request whose preparation
+ * Times: sfmm_server_request_info fills times[5]: arrival and admission
+ * (host clock) and completion (GPU clock), in ms since the server's
+ * creation, the request's own GPU time, and when the loop noticed the
+ * completion (host clock; -1 for a request that never ran); rc is its status (a request whose preparation
  * failed, or that can never fit in the arena, finishes with an error code
  * and does not run).  counters: max in flight, arena peak and size, polls
  * blocked on preparation, slot, arena, hazard.  Destroy drains first. */

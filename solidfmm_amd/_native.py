@@ -53,6 +53,8 @@ SIGNATURES = {
     "sfmm_plan_device_bytes": (_c.c_size_t, [_vp]),
     "sfmm_plan_host_bytes": (_c.c_size_t, [_vp]),
     "sfmm_plan_bind_staged": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
+    "sfmm_plan_copy_tables": (_c.c_int, [_vp, _vp, _c.c_size_t]),
+    "sfmm_plan_bind_pinned": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _vp]),
     "sfmm_device": (_c.c_int, [_vp]),
     "sfmm_server_create": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _c.c_size_t, _c.c_int, _c.c_size_t,
                                       _c.c_int, _c.POINTER(_vp)]),

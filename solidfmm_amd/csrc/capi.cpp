@@ -579,6 +579,31 @@ int sfmm_plan_bind_staged(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void
     return SFMM_OK;
 }
 
+int sfmm_plan_copy_tables(const sfmm_plan *plan, void *host_mem, size_t host_bytes)
+{
+    if (!plan || (!plan->blob.empty() && (!host_mem || host_bytes < plan->blob.size()))) return SFMM_ERR_LOGIC;
+    if (!plan->blob.empty()) std::memcpy(host_mem, plan->blob.data(), plan->blob.size());
+    return SFMM_OK;
+}
+
+int sfmm_plan_bind_pinned(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, const void *host_tables, void *stream)
+{
+    if (!plan) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = plan->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (plan->owns_mem) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan owns its device memory");
+    if (plan->dev_bytes && (!dev_mem || dev_bytes < plan->dev_bytes || (reinterpret_cast<uintptr_t>(dev_mem) & 255u)))
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: device buffer missing, misaligned or too small");
+    if (!plan->blob.empty() && !host_tables) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: host tables missing");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess && !plan->blob.empty())
+        e = hipMemcpyAsync(dev_mem, host_tables, plan->blob.size(), hipMemcpyHostToDevice,
+                           static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipMemcpyAsync");
+    plan->d_mem = static_cast<unsigned char *>(dev_mem);
+    return SFMM_OK;
+}
+
 int sfmm_plan_bind(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void *stream)
 {
     if (!plan) return SFMM_ERR_LOGIC;

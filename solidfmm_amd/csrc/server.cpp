@@ -38,11 +38,16 @@
 #include <thread>
 #include <vector>
 
+#include <sys/resource.h>
+#include <sys/syscall.h>
+#include <unistd.h>
+
 #include "../../include/solidfmm_gpu.h"
 
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr int kPreparerNice = 5;  // niceness of the preparation threads
 
 struct Request {
     uint64_t id = 0;
@@ -52,6 +57,7 @@ struct Request {
     double arrival_ms = 0;                       // host clock since the server's start
     double admitted_ms = -1;
     double done_ms = -1;  // GPU clock since the server's start
+    double noticed_ms = -1;  // host clock: when the loop saw the completion
     double gpu_ms = -1;
     int rc = SFMM_OK;
     bool prepared = false;
@@ -59,6 +65,8 @@ struct Request {
     sfmm_plan *plan = nullptr;
     int slot = -1;
     size_t arena_off = 0, arena_len = 0;
+    bool staged = false;  // tables copied into the pinned pool by the preparer
+    size_t host_off = 0, host_len = 0;
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
 };
 
@@ -120,10 +128,11 @@ struct sfmm_server {
     size_t ncells = 0;
     int device = 0;
     std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> slot_start, slot_end;  // per slot, reused by its requests
     std::vector<int> free_slots;
     void *arena_mem = nullptr;
-    unsigned char *arena_host = nullptr;  // pinned mirror: plan tables pass through it
-    std::unique_ptr<Arena> arena;
+    unsigned char *host_pool = nullptr;  // pinned: plan tables pass through it
+    std::unique_ptr<Arena> arena, host_arena;
     std::vector<int32_t> readers[2], writers[2];
     hipEvent_t ref = nullptr;
     std::chrono::steady_clock::time_point t0;
@@ -168,15 +177,16 @@ struct sfmm_server {
     void finish_locked(Request &r)
     {
         float ms = 0;
+        r.noticed_ms = now_ms();
         if (hipEventElapsedTime(&ms, ref, r.ev_end) == hipSuccess) r.done_ms = ms;
         if (hipEventElapsedTime(&ms, r.ev_start, r.ev_end) == hipSuccess) r.gpu_ms = ms;
         hazards(r, -1);
         arena->release(r.arena_off, r.arena_len);
+        if (r.staged) host_arena->release(r.host_off, r.host_len);
+        r.staged = false;
         free_slots.push_back(r.slot);
         sfmm_plan_destroy(r.plan);
         r.plan = nullptr;
-        (void)hipEventDestroy(r.ev_start);
-        (void)hipEventDestroy(r.ev_end);
         r.ev_start = r.ev_end = nullptr;
         r.finished = true;
         --unfinished;
@@ -188,6 +198,8 @@ struct sfmm_server {
     {
         r.rc = rc;
         r.finished = true;
+        if (r.staged) host_arena->release(r.host_off, r.host_len);
+        r.staged = false;
         if (r.plan) sfmm_plan_destroy(r.plan);
         r.plan = nullptr;
         --unfinished;
@@ -198,13 +210,29 @@ struct sfmm_server {
     {
         const hipStream_t s = streams[size_t(r.slot)];
         const int sp = src_pool_of(r.op), dp = dst_pool_of(r.op);
-        if (hipEventCreate(&r.ev_start) != hipSuccess || hipEventCreate(&r.ev_end) != hipSuccess) return false;
+        r.ev_start = slot_start[size_t(r.slot)];
+        r.ev_end = slot_end[size_t(r.slot)];
         if (hipEventRecord(r.ev_start, s) != hipSuccess) return false;
         void *mem = static_cast<unsigned char *>(arena_mem) + r.arena_off;
-        // the request's range of the pinned mirror stays reserved with its
-        // device range until completion, so the copy reads stable memory
-        if (sfmm_plan_bind_staged(r.plan, mem, r.arena_len, arena_host + r.arena_off, r.arena_len, s) != SFMM_OK)
-            return false;
+        // tables staged by the preparer: only the asynchronous copy is left
+        // (the pinned range stays reserved until completion); otherwise
+        // stage them now, or copy from the plan's pageable memory
+        if (!r.staged) {
+            const size_t hb = sfmm_plan_host_bytes(r.plan);
+            size_t off = 0, len = 0;
+            if (hb && host_arena->alloc(hb, off, len)) {
+                if (sfmm_plan_copy_tables(r.plan, host_pool + off, len) != SFMM_OK) {
+                    host_arena->release(off, len);
+                    return false;
+                }
+                r.staged = true;
+                r.host_off = off;
+                r.host_len = len;
+            }
+        }
+        const int rc = r.staged ? sfmm_plan_bind_pinned(r.plan, mem, r.arena_len, host_pool + r.host_off, s)
+                                : sfmm_plan_bind(r.plan, mem, r.arena_len, s);
+        if (rc != SFMM_OK) return false;
         if (sfmm_plan_execute(r.plan, pool[sp], pool_len[sp], pool[dp], pool_len[dp], s) != SFMM_OK) return false;
         return hipEventRecord(r.ev_end, s) == hipSuccess;
     }
@@ -279,8 +307,6 @@ struct sfmm_server {
                     hazards(r, -1);
                     arena->release(off, len);
                     free_slots.push_back(r.slot);
-                    if (r.ev_start) (void)hipEventDestroy(r.ev_start);
-                    if (r.ev_end) (void)hipEventDestroy(r.ev_end);
                     r.ev_start = r.ev_end = nullptr;
                     fail_locked(r, SFMM_ERR_HIP);
                 } else {
@@ -292,7 +318,9 @@ struct sfmm_server {
             if (stop && waiting.empty() && inflight.empty() && future.empty()) return;
             if (progressed) continue;
             if (!inflight.empty()) {
-                // kernels are running: poll again shortly
+                // kernels are running: poll again shortly (polling without
+                // a pause measured slower: the event queries contend with
+                // the launches in the HIP runtime)
                 lock.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(5));
                 lock.lock();
@@ -308,6 +336,10 @@ struct sfmm_server {
 
     void prepare_loop()
     {
+        // below the scheduler: when the preparers fill every core, a woken
+        // scheduler otherwise waits up to a scheduler tick (milliseconds)
+        // to notice completions and admit requests
+        (void)setpriority(PRIO_PROCESS, static_cast<id_t>(syscall(SYS_gettid)), kPreparerNice);
         std::unique_lock<std::mutex> lock(mu);
         for (;;) {
             cv_prep.wait(lock, [&] { return stop || !to_prepare.empty(); });
@@ -317,9 +349,27 @@ struct sfmm_server {
             lock.unlock();
             sfmm_plan *plan = nullptr;
             const int rc = sfmm_plan_prepare(ctx, r->op, r->items.data(), r->items.size(), &plan);
+            // stage the tables in pinned memory here, off the scheduler
+            size_t off = 0, len = 0;
+            bool staged = false;
+            const size_t hb = rc == SFMM_OK ? sfmm_plan_host_bytes(plan) : 0;
+            if (hb) {
+                lock.lock();
+                staged = host_arena->alloc(hb, off, len);
+                lock.unlock();
+                if (staged && sfmm_plan_copy_tables(plan, host_pool + off, len) != SFMM_OK) {
+                    lock.lock();
+                    host_arena->release(off, len);
+                    lock.unlock();
+                    staged = false;
+                }
+            }
             lock.lock();
             r->plan = plan;
             r->rc = rc;
+            r->staged = staged;
+            r->host_off = off;
+            r->host_len = len;
             r->prepared = true;
             cv_work.notify_one();
         }
@@ -354,23 +404,31 @@ int sfmm_server_create(sfmm_context *ctx, double *m_pool, size_t m_len, double *
         if (e == hipSuccess) {
             sv->streams.push_back(st);
             sv->free_slots.push_back(s);
+            hipEvent_t a = nullptr, b = nullptr;
+            e = hipEventCreate(&a);
+            if (e == hipSuccess) sv->slot_start.push_back(a);
+            if (e == hipSuccess) e = hipEventCreate(&b);
+            if (e == hipSuccess) sv->slot_end.push_back(b);
         }
     }
     if (e == hipSuccess) e = hipMalloc(&sv->arena_mem, arena_bytes);
     if (e == hipSuccess)
-        e = hipHostMalloc(reinterpret_cast<void **>(&sv->arena_host), arena_bytes, hipHostMallocDefault);
+        e = hipHostMalloc(reinterpret_cast<void **>(&sv->host_pool), arena_bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreate(&sv->ref);
     if (e == hipSuccess) e = hipEventRecord(sv->ref, nullptr);
     if (e == hipSuccess) e = hipEventSynchronize(sv->ref);
     if (e != hipSuccess) {
         for (hipStream_t st : sv->streams) (void)hipStreamDestroy(st);
+        for (hipEvent_t ev : sv->slot_start) (void)hipEventDestroy(ev);
+        for (hipEvent_t ev : sv->slot_end) (void)hipEventDestroy(ev);
         if (sv->arena_mem) (void)hipFree(sv->arena_mem);
-        if (sv->arena_host) (void)hipHostFree(sv->arena_host);
+        if (sv->host_pool) (void)hipHostFree(sv->host_pool);
         if (sv->ref) (void)hipEventDestroy(sv->ref);
         return e == hipErrorOutOfMemory ? SFMM_ERR_ALLOC : SFMM_ERR_HIP;
     }
     sv->t0 = std::chrono::steady_clock::now();
     sv->arena.reset(new Arena(arena_bytes));
+    sv->host_arena.reset(new Arena(arena_bytes));
     sfmm_server *raw = sv.get();
     for (int w = 0; w < workers; ++w) raw->preparers.emplace_back([raw] { raw->prepare_loop(); });
     raw->scheduler = std::thread([raw] { raw->schedule(); });
@@ -442,6 +500,7 @@ int sfmm_server_request_info(sfmm_server *sv, uint64_t rid, double *times, int *
     times[1] = r.admitted_ms;
     times[2] = r.done_ms;
     times[3] = r.gpu_ms;
+    times[4] = r.noticed_ms;
     if (rc) *rc = r.rc;
     return SFMM_OK;
 }
@@ -475,8 +534,10 @@ void sfmm_server_destroy(sfmm_server *sv)
         (void)hipStreamSynchronize(st);
         (void)hipStreamDestroy(st);
     }
+    for (hipEvent_t ev : sv->slot_start) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : sv->slot_end) (void)hipEventDestroy(ev);
     (void)hipFree(sv->arena_mem);
-    (void)hipHostFree(sv->arena_host);
+    (void)hipHostFree(sv->host_pool);
     (void)hipEventDestroy(sv->ref);
     delete sv;
 }

@@ -48,7 +48,7 @@ class Request:
     """One translation request: operator, items, cells read / written."""
 
     __slots__ = ("rid", "op", "items", "src_cells", "dst_cells", "arrival", "admitted", "done", "gpu_ms",
-                 "slot", "offset", "plan", "future", "ev_start", "ev_end")
+                 "noticed", "slot", "offset", "plan", "future", "ev_start", "ev_end")
 
     def __init__(self, rid, op, items, src_cells, dst_cells, arrival=0.0):
         self.rid = rid
@@ -60,6 +60,7 @@ class Request:
         self.admitted = None
         self.done = None      # ms after the start, GPU clock
         self.gpu_ms = None    # kernel time of the request's own launches
+        self.noticed = None   # ms after the start when the loop saw completion (host clock)
         self.slot = None
         self.offset = None
         self.plan = None
@@ -282,7 +283,8 @@ class NativeServer:
         h = ctypes.c_void_p()
         rc = self._lib.sfmm_server_create(ctx.handle, pools[0].data_ptr(), pools[0].numel(), pools[1].data_ptr(),
                                           pools[1].numel(), int(ncells), int(slots), int(budget_bytes),
-                                          int(workers or min(16, os.cpu_count() or 1)), ctypes.byref(h))
+                                          int(workers or max(1, min(16, os.cpu_count() or 1) - 2)),
+                                          ctypes.byref(h))
         self._check(rc, ctx.handle, "sfmm_server_create")
         self._h = h
 
@@ -313,7 +315,7 @@ class NativeServer:
         t0 = time.perf_counter()
         self._check(self._lib.sfmm_server_drain(self._h), self.ctx.handle, "sfmm_server_drain")
         wall = time.perf_counter() - t0
-        times = np.zeros(4)
+        times = np.zeros(5)
         status = c.c_int()
         for r, rid in rids:
             self._check(self._lib.sfmm_server_request_info(self._h, rid, times.ctypes.data, c.byref(status)),
@@ -322,6 +324,7 @@ class NativeServer:
             r.done = times[2] - base
             r.gpu_ms = times[3]
             r.admitted = (times[1] - base) * 1e-3
+            r.noticed = times[4] - base
         cnt = np.zeros(7, dtype=np.uint64)
         self._lib.sfmm_server_counters(self._h, cnt.ctypes.data)
         arr = {id(r): (1e3 * r.arrival if realtime else 0.0) for r in requests}
@@ -329,6 +332,8 @@ class NativeServer:
         n = sum(len(r.items) for r in requests)
         busy = sum(r.gpu_ms for r in requests)
         span = max(r.done for r in requests) - min(arr.values())
+        # host clock minus GPU clock (aligned at the server's creation)
+        seen = np.array([r.noticed - r.done for r in requests])
         return {"requests": len(requests), "translations": int(n), "wall_s": wall, "span_ms": span,
                 "translations_per_s": n / (span * 1e-3) if span > 0 else 0.0,
                 "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
@@ -336,7 +341,9 @@ class NativeServer:
                 "max_inflight": int(cnt[0]), "arena_peak_bytes": int(cnt[1]), "arena_bytes": int(cnt[2]),
                 "sum_request_gpu_ms": busy, "overlap": busy / span if span > 0 else 0.0,
                 "blocked_polls": {"prepare": int(cnt[3]), "slot": int(cnt[4]), "arena": int(cnt[5]),
-                                  "hazard": int(cnt[6])}}
+                                  "hazard": int(cnt[6])},
+                "notice_delay_ms": {"p50": float(np.percentile(seen, 50)), "p99": float(np.percentile(seen, 99)),
+                                    "max": float(seen.max())}}
 
     def close(self):
         if getattr(self, "_h", None):

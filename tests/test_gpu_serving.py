@@ -133,7 +133,7 @@ def test_native_server_rejects_and_continues(sfmm):
     ok = [(r, server.submit(r)) for r in reqs]
     lib = server._lib
     assert lib.sfmm_server_drain(server._h) == 0
-    times = np.zeros(4)
+    times = np.zeros(5)
     rc = ctypes.c_int()
     assert lib.sfmm_server_request_info(server._h, rid_bad, times.ctypes.data, ctypes.byref(rc)) == 0
     assert rc.value == sfmm.SFMM_ERR_RANGE if hasattr(sfmm, "SFMM_ERR_RANGE") else rc.value == -2
@@ -141,7 +141,7 @@ def test_native_server_rejects_and_continues(sfmm):
     assert rc.value == -4
     for r, rid in ok:
         assert lib.sfmm_server_request_info(server._h, rid, times.ctypes.data, ctypes.byref(rc)) == 0
-        assert rc.value == 0 and times[2] >= 0
+        assert rc.value == 0 and times[2] >= 0 and times[4] >= times[1]
     server.close()
     for a, b in zip(alone, mix):
         assert torch.equal(a, b)
