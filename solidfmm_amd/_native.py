@@ -285,6 +285,26 @@ class Plan:
         _check(load().sfmm_plan_bind(self._h, ptr, nbytes, _stream_ptr(stream)), self.ctx.handle, "sfmm_plan_bind")
 
     @property
+    def host_bytes(self):
+        """Bytes of the tables a bind copies (sfmm_plan_host_bytes)."""
+        return load().sfmm_plan_host_bytes(self._h)
+
+    def bind_staged(self, ptr, nbytes, host_ptr, host_nbytes, stream=None):
+        """bind() through caller pinned host memory (sfmm_plan_bind_staged)."""
+        _check(load().sfmm_plan_bind_staged(self._h, ptr, nbytes, host_ptr, host_nbytes, _stream_ptr(stream)),
+               self.ctx.handle, "sfmm_plan_bind_staged")
+
+    def copy_tables(self, host_ptr, host_nbytes):
+        """Host half of bind_staged (sfmm_plan_copy_tables)."""
+        _check(load().sfmm_plan_copy_tables(self._h, host_ptr, host_nbytes), self.ctx.handle,
+               "sfmm_plan_copy_tables")
+
+    def bind_pinned(self, ptr, nbytes, host_ptr, stream=None):
+        """Device half of bind_staged (sfmm_plan_bind_pinned)."""
+        _check(load().sfmm_plan_bind_pinned(self._h, ptr, nbytes, host_ptr, _stream_ptr(stream)), self.ctx.handle,
+               "sfmm_plan_bind_pinned")
+
+    @property
     def launches(self):
         return load().sfmm_plan_launches(self._h)
 

@@ -581,7 +581,11 @@ int sfmm_plan_bind_staged(sfmm_plan *plan, void *dev_mem, size_t dev_bytes, void
 
 int sfmm_plan_copy_tables(const sfmm_plan *plan, void *host_mem, size_t host_bytes)
 {
-    if (!plan || (!plan->blob.empty() && (!host_mem || host_bytes < plan->blob.size()))) return SFMM_ERR_LOGIC;
+    if (!plan) return SFMM_ERR_LOGIC;
+    if (!plan->blob.empty() && (!host_mem || host_bytes < plan->blob.size())) {
+        std::lock_guard<std::mutex> lock(plan->ctx->mu);
+        return fail(plan->ctx, SFMM_ERR_LOGIC, "solidfmm: host staging missing or too small");
+    }
     if (!plan->blob.empty()) std::memcpy(host_mem, plan->blob.data(), plan->blob.size());
     return SFMM_OK;
 }
