@@ -539,7 +539,7 @@ def main():
     ap.add_argument("--points", type=int, default=1_000_000, help="serving: Plummer points")
     ap.add_argument("--load", type=float, default=0.8,
                     help="serving: Poisson arrival rate as a fraction of the measured one-at-a-time capacity")
-    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=6)
     ap.add_argument("--budget-mib", type=int, default=1024)
     ap.add_argument("--precisions", default="fp64,split32")
     ap.add_argument("--pmin", type=int, default=4)

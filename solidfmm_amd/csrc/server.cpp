@@ -318,9 +318,11 @@ struct sfmm_server {
             if (stop && waiting.empty() && inflight.empty() && future.empty()) return;
             if (progressed) continue;
             if (!inflight.empty()) {
-                // kernels are running: poll again shortly (polling without
-                // a pause measured slower: the event queries contend with
-                // the launches in the HIP runtime)
+                // kernels are running: poll again shortly.  The pause lasts
+                // about the timer slack (50 us); polling without a pause, or
+                // with a 1 us slack, measured slower, with a longer latency
+                // tail: the event queries contend with the launches in the
+                // HIP runtime
                 lock.unlock();
                 std::this_thread::sleep_for(std::chrono::microseconds(5));
                 lock.lock();
