@@ -61,6 +61,21 @@ constexpr int nslots()
     return (P + 3) / 4;
 }
 
+// Orders whose slot 0 of the rotated triangle (PI reals and PI imaginary
+// parts per lane) lives in LDS instead of registers: from 17 on the
+// register file overflows into scratch memory, and up to 22 the LDS of a
+// workgroup holds the swap table plus kWaves such slots.
+template <int PI>
+constexpr bool lds_slot0()
+{
+    return PI >= 17 && PI <= 22;
+}
+template <int PI>
+constexpr int lds_slot0_size()  // doubles per wave
+{
+    return lds_slot0<PI>() ? 2 * PI * 64 : 0;
+}
+
 __device__ __forceinline__ void sched_fence() { __builtin_amdgcn_sched_barrier(0); }
 
 __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c)
@@ -204,8 +219,11 @@ __device__ __forceinline__ uint32_t wave_max7(uint32_t v)
 template <int OP, int PI, int PO, bool RT>
 __device__ __forceinline__ void translate_group(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                                                 const sfmm_item *__restrict__ items, uint32_t n_items, uint32_t g,
-                                                const double *tab, int lane, int mode)
+                                                const double *tab, double *tl, int lane, int mode)
 {
+    // tl: this wave's LDS copy of slot 0 of the triangle when
+    // lds_slot0<PI>(): Re of degree K at tl[64 K + lane], Im at
+    // tl[64 (PI + K) + lane]; each lane reads back only what it wrote.
     // mode 1: target += T(source); 0: scratch block = T(source); 2: as 0,
     // with the items of a run (consecutive translations of the group with
     // one scratch block) added across lanes and stored by the run's first
@@ -305,8 +323,32 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
         });
     }
 
-    // Rotated triangle, this lane's orders only.
+    // Rotated triangle, this lane's orders only (slot 0 in LDS when
+    // lds_slot0<PI>(); its register copies are then never touched).
     double tr[tri_size<PI>()], ti[tri_size<PI>()];
+    auto put_tri = [&](auto sc, auto Kc, double re, double im) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int K = decltype(Kc)::value;
+        if constexpr (s == 0 && lds_slot0<PI>()) {
+            tl[64 * K + lane] = re;
+            tl[64 * (PI + K) + lane] = im;
+        } else {
+            tr[slot_off<PI>(s) + K - 4 * s] = re;
+            ti[slot_off<PI>(s) + K - 4 * s] = im;
+        }
+    };
+    auto tri_re = [&](auto sc, auto Kc) -> double {
+        constexpr int s = decltype(sc)::value;
+        constexpr int K = decltype(Kc)::value;
+        if constexpr (s == 0 && lds_slot0<PI>()) return tl[64 * K + lane];
+        else return tr[slot_off<PI>(s) + K - 4 * s];
+    };
+    auto tri_im = [&](auto sc, auto Kc) -> double {
+        constexpr int s = decltype(sc)::value;
+        constexpr int K = decltype(Kc)::value;
+        if constexpr (s == 0 && lds_slot0<PI>()) return tl[64 * (PI + K) + lane];
+        else return ti[slot_off<PI>(s) + K - 4 * s];
+    };
 
     auto load_degree = [&](auto nc, double(&xr)[NSL], double(&xi)[NSL]) {
         constexpr int n = decltype(nc)::value;
@@ -330,20 +372,14 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
         constexpr int n = decltype(nc)::value;
         static_for<0, nslots<PI>()>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
-            if constexpr (4 * s <= n) {
-                tr[slot_off<PI>(s) + n - 4 * s] = xr[s];
-                ti[slot_off<PI>(s) + n - 4 * s] = xi[s];
-            }
+            if constexpr (4 * s <= n) put_tri(sc, nc, xr[s], xi[s]);
         });
     };
     auto clear_degree = [&](auto nc) {
         constexpr int n = decltype(nc)::value;
         static_for<0, nslots<PI>()>([&](auto sc) {
             constexpr int s = decltype(sc)::value;
-            if constexpr (4 * s <= n) {
-                tr[slot_off<PI>(s) + n - 4 * s] = 0.0;
-                ti[slot_off<PI>(s) + n - 4 * s] = 0.0;
-            }
+            if constexpr (4 * s <= n) put_tri(sc, nc, 0.0, 0.0);
         });
     };
     auto fetch_old = [&](auto Nc, double(&orr)[NSL], double(&oi)[NSL]) {
@@ -403,6 +439,13 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
     };
 
     if constexpr (OP == SFMM_M2L) {
+        // Output degrees run ascending in the per-item-order kernels, with the
+        // backward scale r^-N as one running product; a table of PO powers
+        // would hold PO registers through the loop (order 17 spilled).  The
+        // fixed-order kernels keep descending degrees and the table: the
+        // compiler schedules their straight-line code with fewer registers
+        // that way (ascending spilled from order 14 on).
+        constexpr bool kUp = RT;
         double sc = rinv;  // r^-(n+1) (translations.cpp:45)
         load_degree(ic<0>{}, pr, pim);
         static_for<0, PI>([&](auto nc) {
@@ -411,7 +454,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             take(xr, pr);
             take(xi, pim);
             if constexpr (n + 1 < PI) load_degree(ic<n + 1>{}, pr, pim);
-            else fetch_old(ic<PO - 1>{}, orr, oi);
+            else fetch_old(ic<kUp ? 0 : PO - 1>{}, orr, oi);
             sched_fence();
             if (!RT || uint32_t(n) < pin_w) {
                 transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
@@ -421,18 +464,13 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             }
             sc *= rinv;
         });
-        double rp[PO];
-        rp[0] = 1.0;
-        static_for<1, PO>([&](auto Nc) {
-            constexpr int N = decltype(Nc)::value;
-            rp[N] = rp[N - 1] * rinv;
-        });
-        static_for_down<0, PO>([&](auto Nc) {
+        auto out_degree = [&](auto Nc, double scale) {
             constexpr int N = decltype(Nc)::value;
             double cr[NSL], ci[NSL];
             take(cr, orr);
             take(ci, oi);
-            if constexpr (N > 0) fetch_old(ic<N - 1>{}, orr, oi);
+            if constexpr (kUp && N + 1 < PO) fetch_old(ic<N + 1>{}, orr, oi);
+            if constexpr (!kUp && N > 0) fetch_old(ic<N - 1>{}, orr, oi);
             sched_fence();
             if (!RT || uint32_t(N) < pout_w) {
                 double xr[NSL], xi[NSL];
@@ -444,8 +482,8 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                         static_for<4 * s, PI>([&](auto Kc) {
                             constexpr int K = decltype(Kc)::value;
                             constexpr double f = kFac.fac[N + K];
-                            ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
-                            ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
+                            ar = __builtin_fma(f, tri_re(sc, Kc), ar);
+                            ai = __builtin_fma(f, tri_im(sc, Kc), ai);
                         });
                         ar = odd ? -ar : ar;
                         ai = odd ? -ai : ai;
@@ -455,10 +493,26 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                     xr[s] = ar;
                     xi[s] = ai;
                 });
-                transform_bwd<N, true>(tab, lane, R, rp[N], xr, xi);
+                transform_bwd<N, true>(tab, lane, R, scale, xr, xi);
                 store_sum(Nc, cr, ci, xr, xi);
             }
-        });
+        };
+        // backward scale r^-N: the same running products either way
+        if constexpr (kUp) {
+            double rpN = 1.0;
+            static_for<0, PO>([&](auto Nc) {
+                out_degree(Nc, rpN);
+                rpN *= rinv;
+            });
+        } else {
+            double rp[PO];
+            rp[0] = 1.0;
+            static_for<1, PO>([&](auto Nc) {
+                constexpr int N = decltype(Nc)::value;
+                rp[N] = rp[N - 1] * rinv;
+            });
+            static_for_down<0, PO>([&](auto Nc) { out_degree(Nc, rp[decltype(Nc)::value]); });
+        }
     } else if constexpr (OP == SFMM_M2M) {
         double sc = rinv;  // forward r^-(n+1)
         double sb = r;     // backward r^(N+1) (translations.cpp:129)
@@ -498,8 +552,8 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                             static_for<4 * s, KE>([&](auto Kc) {
                                 constexpr int K = decltype(Kc)::value;
                                 constexpr double f = kFac.inv[N - K];
-                                ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
-                                ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
+                                ar = __builtin_fma(f, tri_re(scc, Kc), ar);
+                                ai = __builtin_fma(f, tri_im(scc, Kc), ai);
                             });
                             ar = (q + 4 * s > N) ? 0.0 : ar;
                             ai = (q + 4 * s > N) ? 0.0 : ai;
@@ -560,8 +614,8 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                             static_for<N, PI>([&](auto Kc) {
                                 constexpr int K = decltype(Kc)::value;
                                 constexpr double f = kFac.inv[K - N];
-                                ar = __builtin_fma(f, tr[slot_off<PI>(s) + K - 4 * s], ar);
-                                ai = __builtin_fma(f, ti[slot_off<PI>(s) + K - 4 * s], ai);
+                                ar = __builtin_fma(f, tri_re(scc, Kc), ar);
+                                ai = __builtin_fma(f, tri_im(scc, Kc), ai);
                             });
                             ar = (q + 4 * s > N) ? 0.0 : ar;
                             ai = (q + 4 * s > N) ? 0.0 : ai;
@@ -595,11 +649,14 @@ __global__ void __launch_bounds__(64 * kWaves)
     constexpr int PM = PI > PO ? PI : PO;
     constexpr int TAB = dense_size(PM);
     __shared__ double tab[TAB];
+    constexpr int TL = lds_slot0_size<PI>();
+    __shared__ double tri0[TL > 0 ? kWaves * TL : 1];
     for (int i = threadIdx.x; i < TAB; i += 64 * kWaves) tab[i] = table[i];
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
     const uint32_t wave = threadIdx.x >> 6;
+    double *tl = tri0 + wave * TL;
     const uint32_t groups = (n_items + 15u) / 16u;
     for (uint32_t g = blockIdx.x * kWaves + wave; g < groups; g += gridDim.x * kWaves) {
         // The table is the same for every group: an opaque zero offset per
@@ -608,7 +665,7 @@ __global__ void __launch_bounds__(64 * kWaves)
         // each group reads its operands from LDS where they are used.
         int base = 0;
         asm volatile("" : "+s"(base));
-        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab + base, lane, acc);
+        translate_group<OP, PI, PO, RT>(src_pool, dst_pool, items, n_items, g, tab + base, tl, lane, acc);
     }
 }
 
