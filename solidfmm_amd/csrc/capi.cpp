@@ -754,7 +754,11 @@ int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, v
     if (op < SFMM_M2M || op > SFMM_L2L) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: unknown operator");
     if (n && !items) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null item list");
     if (ctx->precision != SFMM_FP64) return SFMM_OK;  // one kernel per case: nothing to choose
-    constexpr size_t kSample = 32768;
+    // large enough that every family fills the GPU (256k items: four waves
+    // per SIMD for the one-lane-per-translation kernels); a 32k sample left
+    // those at half a wave per SIMD and picked MFMA at order 12, 38 % slower
+    // on the FMM's 295k-item bucket
+    constexpr size_t kSample = size_t(1) << 18;
     constexpr int kReps = 5;
     std::vector<std::vector<sfmm_item>> bucket(kOrders);
     for (size_t i = 0; i < n; ++i) {
@@ -796,40 +800,58 @@ int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, v
         if (e == hipSuccess) e = hipEventCreate(&ev[1]);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "autotune setup");
     }
+    // Each family is timed on the plan it would get for the bucket's sample:
+    // the sample keeps the list's order and which of its items share a
+    // target (targets remapped to a compact pool, sources to distinct
+    // blocks), so repeated targets are timed in scratch mode with their runs
+    // and the reduction, as the real list runs.
+    std::unordered_map<uint64_t, uint64_t> tid;
     for (uint32_t pb = 1; rc == SFMM_OK && pb < uint32_t(kOrders); ++pb) {
         std::vector<sfmm_item> &b = bucket[pb];
         if (b.empty()) continue;
-        std::stable_sort(b.begin(), b.end(), [](const sfmm_item &x, const sfmm_item &y) {
-            return x.p_in != y.p_in ? x.p_in < y.p_in : x.p_out < y.p_out;
-        });
-        bool rt = false;
+        tid.clear();
         for (size_t i = 0; i < b.size(); ++i) {
             b[i].src = i * block;
-            b[i].dst = i * block;
-            rt = rt || b[i].p_in != pb || b[i].p_out != pb;
-        }
-        e = hipMemcpyAsync(d_items, b.data(), b.size() * sizeof(sfmm_item), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) {
-            rc = hip_fail(ctx, e, "autotune upload");
-            break;
+            b[i].dst = tid.emplace(b[i].dst, uint64_t(tid.size())).first->second * block;
         }
         int best = 0;
         float best_ms = 0.0f;
         for (int fam : {int(sfmm::kLaneKernels), int(sfmm::kMfmaKernels), int(sfmm::kGenericKernel)}) {
             if (!sfmm::family_covers(fam, pb)) continue;
-            const sfmm::KernelChoice k{fam, pb, rt};
-            e = sfmm::launch_kernel(op, k, d_items, b.size(), src, dst, s);  // warm-up
+            sfmm_context view;
+            view.device = ctx->device;
+            view.P = ctx->P;
+            view.family = SFMM_KERNELS_AUTO;
+            view.precision = ctx->precision;
+            std::memcpy(view.tune, ctx->tune, sizeof(view.tune));
+            view.tune[op][pb] = fam;
+            sfmm_plan *plan = nullptr;
+            rc = prepare_plan(&view, op, b.data(), b.size(), &plan);
+            if (rc != SFMM_OK) {
+                ctx->err = view.err;
+                break;
+            }
+            plan->ctx = ctx;
+            e = hipSuccess;
+            if (plan->dev_bytes) {
+                e = hipMalloc(&plan->d_mem, plan->dev_bytes);
+                if (e != hipSuccess) plan->d_mem = nullptr;
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(plan->d_mem, plan->blob.data(), plan->blob.size(), hipMemcpyHostToDevice, s);
+            }
+            if (e == hipSuccess) e = enqueue(plan, src, dst, s);  // warm-up
             float ms = 1e30f;
             for (int r = 0; r < kReps && e == hipSuccess; ++r) {
                 e = hipEventRecord(ev[0], s);
-                if (e == hipSuccess) e = sfmm::launch_kernel(op, k, d_items, b.size(), src, dst, s);
+                if (e == hipSuccess) e = enqueue(plan, src, dst, s);
                 if (e == hipSuccess) e = hipEventRecord(ev[1], s);
                 if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
                 float t = 0.0f;
                 if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
                 ms = std::min(ms, t);
             }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            sfmm_plan_destroy(plan);
             if (e != hipSuccess) {
                 rc = hip_fail(ctx, e, "autotune launch");
                 break;
