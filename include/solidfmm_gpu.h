@@ -184,9 +184,13 @@ int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, d
 int sfmm_plan_replay(sfmm_plan *plan, void *stream);
 
 /* Autotuner: for each order bucket max(p_in, p_out) present in the list,
- * time every kernel family that covers it on a sample of the bucket's items
- * (scratch pools, on `stream`; blocks until done) and keep the fastest for
- * plans created afterwards on this context.  sfmm_get_tuning reports the
+ * time the plan every kernel family that covers it would get for a sample
+ * of the bucket's items (up to 262,144 in list order, with the list's
+ * repeated targets, so runs and reductions are timed as they will run) and
+ * keep the fastest for plans created afterwards on this context.  The
+ * sample runs in scratch pools the call allocates and frees (two of
+ * sample x P(P+1) doubles: 1.8 GB at order 20), on `stream`; it blocks
+ * until done.  sfmm_get_tuning reports the
  * family used per order bucket (index = order, entry 0 unused);
  * sfmm_set_tuning sets one entry (SFMM_KERNELS_AUTO restores the default).
  * Plays the role of the reference's schedule_by_measuring_throughput
