@@ -188,9 +188,9 @@ int sfmm_plan_replay(sfmm_plan *plan, void *stream);
  * of the bucket's items (up to 262,144 in list order, with the list's
  * repeated targets, so runs and reductions are timed as they will run) and
  * keep the fastest for plans created afterwards on this context.  The
- * sample runs in scratch pools the call allocates and frees (two of
- * sample x P(P+1) doubles: 1.8 GB at order 20), on `stream`; it blocks
- * until done.  sfmm_get_tuning reports the
+ * sample runs in scratch pools the call allocates and frees (two of the
+ * largest sample x P(P+1) doubles over the buckets: 1.8 GB for a full
+ * sample at order 20), on `stream`; it blocks until done.  sfmm_get_tuning reports the
  * family used per order bucket (index = order, entry 0 unused);
  * sfmm_set_tuning sets one entry (SFMM_KERNELS_AUTO restores the default).
  * Plays the role of the reference's schedule_by_measuring_throughput

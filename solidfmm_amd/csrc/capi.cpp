@@ -774,28 +774,23 @@ int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, v
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     if (!tables_ready()) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: device tables");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    size_t most = 0;
-    uint32_t pmax = 1;
+    // pools for the largest bucket: one block of the bucket's order per
+    // sample item
+    size_t reals = 0;
     for (uint32_t pb = 1; pb < uint32_t(kOrders); ++pb)
-        if (!bucket[pb].empty()) {
-            most = std::max(most, bucket[pb].size());
-            pmax = pb;
-        }
-    if (most == 0) return SFMM_OK;
-    const size_t block = size_t(pmax) * (pmax + 1);
+        reals = std::max(reals, bucket[pb].size() * size_t(pb) * (pb + 1));
+    if (reals == 0) return SFMM_OK;
     double *src = nullptr, *dst = nullptr;
-    sfmm_item *d_items = nullptr;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int rc = SFMM_OK;
-    e = hipMalloc(&src, most * block * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&dst, most * block * sizeof(double));
-    if (e == hipSuccess) e = hipMalloc(&d_items, most * sizeof(sfmm_item));
+    e = hipMalloc(&src, reals * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dst, reals * sizeof(double));
     if (e != hipSuccess) {
         rc = fail(ctx, SFMM_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
     } else {
         // 0x3F bytes: every double is ~4.8e-4 (normal numbers)
-        e = hipMemsetAsync(src, 0x3F, most * block * sizeof(double), s);
-        if (e == hipSuccess) e = hipMemsetAsync(dst, 0x3F, most * block * sizeof(double), s);
+        e = hipMemsetAsync(src, 0x3F, reals * sizeof(double), s);
+        if (e == hipSuccess) e = hipMemsetAsync(dst, 0x3F, reals * sizeof(double), s);
         if (e == hipSuccess) e = hipEventCreate(&ev[0]);
         if (e == hipSuccess) e = hipEventCreate(&ev[1]);
         if (e != hipSuccess) rc = hip_fail(ctx, e, "autotune setup");
@@ -809,6 +804,7 @@ int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, v
     for (uint32_t pb = 1; rc == SFMM_OK && pb < uint32_t(kOrders); ++pb) {
         std::vector<sfmm_item> &b = bucket[pb];
         if (b.empty()) continue;
+        const size_t block = size_t(pb) * (pb + 1);
         tid.clear();
         for (size_t i = 0; i < b.size(); ++i) {
             b[i].src = i * block;
@@ -867,7 +863,6 @@ int sfmm_autotune(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, v
     if (ev[1]) (void)hipEventDestroy(ev[1]);
     if (src) (void)hipFree(src);
     if (dst) (void)hipFree(dst);
-    if (d_items) (void)hipFree(d_items);
     return rc;
 }
 
