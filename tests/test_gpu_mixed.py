@@ -166,6 +166,44 @@ def test_heavily_repeated_targets(sfmm, orc, op):
     assert np.array_equal(again, got)  # bitwise reproducible
 
 
+@pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
+def test_autotune_on_repeated_targets(sfmm, orc, op):
+    """The autotuner times the plan each family would get, with the list's
+    repeated targets (scratch blocks, runs, reduction), on scratch pools
+    sized per bucket: one bucket of 3,000 items into 7 targets of order 12
+    beside 24 independent translations of order 20.  The tuned plan stays
+    within tolerance of the oracle and uses the tuned families."""
+    import oracle
+    from tests.helpers import assert_parity
+    n, nt, P = 3000, 7, 12
+    a, src_a, _ = W.mixed_batch(n, 0xAC + op, 4, P, op=op)
+    a["p_out"] = P
+    blk = P * (P + 1)
+    a["dst"] = ((W.uniform01(0x57, op, n) * nt).astype(np.int64) * blk).astype(np.uint64)
+    shift = "m2l" if op == W.M2L else "octant"
+    b, src_b, _ = W.uniform_batch(op, 20, 20, 24, 0xAD + op, shift=shift)
+    b["src"] += np.uint64(len(src_a))
+    b["dst"] += np.uint64(nt * blk)
+    items = np.concatenate([a, b])
+    src = np.concatenate([src_a, src_b])
+    dst0 = W.uniform(0x67, op, nt * blk + 24 * 420, -1.0, 1.0)
+    ctx = sfmm.Context(20)
+    ctx.autotune(op, items)
+    tuning = ctx.tuning(op)
+    plan = sfmm.Plan(ctx, op, items)
+    for fam, order, rt, count in plan.launch_info():
+        if fam != GENERIC:  # generic launches mix buckets
+            assert fam == tuning[order]
+    d = dev(dst0)
+    plan.execute(dev(src), d)
+    torch.cuda.synchronize()
+    got = d.cpu().numpy()
+    want = orc.translate(op, items, src, dst0.copy())
+    e = oracle.degree_errors(got[:nt * blk], want[:nt * blk], P).max(axis=1)
+    assert e.max() <= 1e-11, e.max()
+    assert_parity(got[nt * blk:], want[nt * blk:], 20, 20, label=f"op={op} order 20")
+
+
 def test_prepared_plan_matches_created(sfmm):
     items, src, _ = W.mixed_batch(4000, 0x99, 4, 16)
     items["dst"] = (np.arange(4000) % 50 * 272).astype(np.uint64)  # 80 items per target
