@@ -281,8 +281,14 @@ __device__ __forceinline__ void segment_sum(const Segments &g, double (&re)[N + 
 // target += T(source); acc = 0: target = T(source) over the degrees the
 // translation defines (scratch blocks of a segmented reduction); acc = 2:
 // as 0, with the lanes of a run adding into one block (Segments).
+// Waves per SIMD the register allocation must allow.  M2L at order 7 needs
+// 264 registers unconstrained, just over the 256 that two waves per SIMD
+// leave each lane; held to 256 it keeps two waves in flight.
+template <int OP, int PI>
+inline constexpr int kLaneWaves = (OP == SFMM_M2L && PI == 7) ? 2 : 1;
+
 template <int OP, int PI, int PO, bool RT>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLaneWaves<OP, PI>)))
     k_translate(const double *__restrict__ src_pool, double *__restrict__ dst_pool,
                 const sfmm_item *__restrict__ items, uint32_t n_items, int acc)
 {
