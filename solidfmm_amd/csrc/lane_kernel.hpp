@@ -285,7 +285,7 @@ __device__ __forceinline__ void segment_sum(const Segments &g, double (&re)[N + 
 // 264 registers unconstrained, just over the 256 that two waves per SIMD
 // leave each lane; held to 256 it keeps two waves in flight.
 template <int OP, int PI>
-inline constexpr int kLaneWaves = (OP == SFMM_M2L && PI == 7) ? 2 : 1;
+inline constexpr int kLaneWaves = (PI == 7 || PI == 8) ? 2 : 1;
 
 template <int OP, int PI, int PO, bool RT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kLaneWaves<OP, PI>)))
