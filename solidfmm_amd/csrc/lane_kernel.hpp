@@ -281,9 +281,11 @@ __device__ __forceinline__ void segment_sum(const Segments &g, double (&re)[N + 
 // target += T(source); acc = 0: target = T(source) over the degrees the
 // translation defines (scratch blocks of a segmented reduction); acc = 2:
 // as 0, with the lanes of a run adding into one block (Segments).
-// Waves per SIMD the register allocation must allow.  M2L at order 7 needs
-// 264 registers unconstrained, just over the 256 that two waves per SIMD
-// leave each lane; held to 256 it keeps two waves in flight.
+// Waves per SIMD the register allocation must allow.  At orders 7 and 8 the
+// kernels need 256-316 registers unconstrained; held to the 256 that two
+// waves per SIMD leave each lane (spilling up to 59 of them at order 8),
+// they keep two waves in flight and run 19-26 % faster.  At orders 9 and
+// 10 the spills (113-173 registers) cost more than the second wave gains.
 template <int OP, int PI>
 inline constexpr int kLaneWaves = (PI == 7 || PI == 8) ? 2 : 1;
 
