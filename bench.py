@@ -85,6 +85,31 @@ def cpu_baseline(op, items, src, gpu_out, P, threads):
     return {"value": n / secs, "unit": "translations/s", "cores": cores, "kind": kind, "sample": note}, err
 
 
+def truth_errors(op, items, src, gpu_out, P, nchk=4096):
+    """Per-item errors (max over degrees of the per-degree normwise relative
+    error) of the GPU, the compiled reference and the FP64 oracle against
+    the binary128 truth (oracle/truth.c) on the batch's first nchk items."""
+    import oracle
+    opi = OPS[op]
+    nchk = min(nchk, len(items))
+    sub = items[:nchk].copy()
+    sub["dst"] = np.arange(nchk, dtype=np.uint64) * np.uint64(P * (P + 1))
+    dst = np.zeros(nchk * P * (P + 1))
+    hi, lo = oracle.Truth().translate(opi, sub, src, len(dst))
+    outs = {"gpu": gpu_out.reshape(len(items), -1)[:nchk].reshape(-1),
+            "oracle": oracle.Oracle().translate(opi, sub, src, np.zeros_like(dst))}
+    if oracle.Reference.available():
+        r = np.zeros_like(dst)
+        ref = oracle.Reference()
+        ref.translate(opi, P, sub, src, r, threads=1, repeats=1, chunk=ref.SAFE_CHUNK)
+        outs["reference"] = r
+    rep = {"items": nchk}
+    for k, v in outs.items():
+        e = oracle.truth_degree_errors(v, hi, lo, P).max(axis=1)
+        rep[k] = {"p50": float(np.percentile(e, 50)), "p99": float(np.percentile(e, 99)), "max": float(e.max())}
+    return rep
+
+
 def copy_mode(args):
     """Plain device-to-device copy of known size: reads and writes
     args.copy_mib MiB per launch."""
@@ -679,6 +704,7 @@ def main():
         line["rel_err_vs_reference"] = {"p50": float(np.percentile(per_item, 50)),
                                         "p99": float(np.percentile(per_item, 99)),
                                         "max": float(per_item.max())}
+        line["rel_err_vs_truth"] = truth_errors(op, items, src, gpu_out, P)
     print(json.dumps(line), flush=True)
 
 

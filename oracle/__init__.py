@@ -1,7 +1,9 @@
 """ctypes bindings of the CPU checkers -- TEST INFRASTRUCTURE ONLY.
 
 ``Oracle`` wraps liboracle.so, the FP64 restatement of the reference
-algorithm (oracle.c).  ``Reference`` wraps _ref/libsolidfmm_ref.so, the
+algorithm (oracle.c).  ``Truth`` wraps libtruth.so, the same operators in
+binary128 by two independent routes (truth.c), against which FP64 results'
+errors are measured.  ``Reference`` wraps _ref/libsolidfmm_ref.so, the
 reference library compiled from its own sources (Makefile, ref_shim.cpp);
 it exists only where it was built.
 
@@ -19,6 +21,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+TRUTH_LIB = os.path.join(HERE, "libtruth.so")
 REF_LIB = os.path.join(HERE, "_ref", "libsolidfmm_ref.so")
 REF_TREE = "/root/reference"
 
@@ -96,6 +99,52 @@ class Oracle:
 
     def dot(self, a, PA, b, PB):
         return self.lib.orc_dot(_ptr(a), PA, _ptr(b), PB)
+
+
+class Truth:
+    """Binary128 evaluation of the operators (truth.c), results as hi + lo."""
+
+    NAIVE, ROTATION = 0, 1
+
+    def __init__(self, path=TRUTH_LIB):
+        if not os.path.exists(path):
+            build(reference=False)
+        self.lib = ctypes.CDLL(path)
+        L = self.lib
+        L.truth_translate.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, _dp, ctypes.c_void_p,
+                                      ctypes.c_size_t, _dp, _dp, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.truth_harmonic.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, _dp, _dp]
+
+    def translate(self, op, items, src, dst_len, dst0=None, method=ROTATION, threads=0, magnitude=False):
+        """(hi, lo) of dst0 + T(src) over the list, in list order per target.
+
+        magnitude=True also returns M, the rotation route run on magnitudes
+        (|A| |v| for every linear step): u * M is the first-order scale of
+        the rounding error of any FP64 evaluation of the algorithm."""
+        items = np.ascontiguousarray(items)
+        src = np.ascontiguousarray(src, dtype=np.float64)
+        hi = np.zeros(dst_len)
+        lo = np.zeros(dst_len)
+        d0 = None
+        if dst0 is not None:
+            d0 = np.ascontiguousarray(dst0, dtype=np.float64)
+            assert len(d0) >= dst_len
+        mag = np.zeros(dst_len) if magnitude else None
+        threads = threads or min(16, len(os.sched_getaffinity(0)))
+        rc = self.lib.truth_translate(op, items.ctypes.data, len(items), _ptr(src),
+                                      None if d0 is None else d0.ctypes.data, dst_len, _ptr(hi), _ptr(lo),
+                                      None if mag is None else mag.ctypes.data, method, threads)
+        if rc != 0:
+            raise RuntimeError(f"truth translate failed ({rc})")
+        return (hi, lo, mag) if magnitude else (hi, lo)
+
+    def harmonic(self, kind, P, x, y, z):
+        hi = np.zeros(P * (P + 1))
+        lo = np.zeros(P * (P + 1))
+        if self.lib.truth_harmonic({"R": 0, "S": 1}[kind], P, x, y, z, _ptr(hi), _ptr(lo)) != 0:
+            raise ValueError("bad harmonic request")
+        return hi, lo
 
 
 class Reference:
@@ -187,6 +236,47 @@ def degree_errors(got, ref, P):
         r = np.linalg.norm(ref[:, idx], axis=1)
         out[:, n] = np.where(r > 0, d / np.where(r > 0, r, 1.0), d)
     return out
+
+
+def truth_degree_errors(got, hi, lo, P):
+    """degree_errors against a binary128 truth given as hi + lo: the
+    difference is formed as (got - hi) - lo, so the truth's own rounding to
+    FP64 does not enter the error."""
+    got = np.asarray(got).reshape(-1, P * (P + 1))
+    hi = np.asarray(hi).reshape(-1, P * (P + 1))
+    lo = np.asarray(lo).reshape(-1, P * (P + 1))
+    out = np.zeros((got.shape[0], P))
+    for n in range(P):
+        idx = slice(n * (n + 1), (n + 1) * (n + 2))
+        d = np.linalg.norm((got[:, idx] - hi[:, idx]) - lo[:, idx], axis=1)
+        r = np.linalg.norm(hi[:, idx], axis=1)
+        out[:, n] = np.where(r > 0, d / np.where(r > 0, r, 1.0), d)
+    return out
+
+
+def error_scale(mag, hi, P):
+    """Per-item, per-degree first-order rounding scale of the rotation
+    algorithm: u ||M_n||_2 / ||truth_n||_2 with M the magnitude propagation
+    (Truth.translate(magnitude=True)) and u = 2^-53."""
+    mag = np.asarray(mag).reshape(-1, P * (P + 1))
+    hi = np.asarray(hi).reshape(-1, P * (P + 1))
+    out = np.zeros((mag.shape[0], P))
+    for n in range(P):
+        idx = slice(n * (n + 1), (n + 1) * (n + 2))
+        m = np.linalg.norm(mag[:, idx], axis=1)
+        r = np.linalg.norm(hi[:, idx], axis=1)
+        out[:, n] = 2.0 ** -53 * np.where(r > 0, m / np.where(r > 0, r, 1.0), m)
+    return out
+
+
+def mixed_truth_errors(got, hi, lo, items):
+    """Per-item error against the truth (max over degrees) for mixed orders."""
+    e = np.zeros(len(items))
+    for po in np.unique(items["p_out"]):
+        idx = np.nonzero(items["p_out"] == po)[0]
+        cols = items["dst"][idx].astype(np.int64)[:, None] + np.arange(int(po) * (int(po) + 1))
+        e[idx] = truth_degree_errors(got[cols], hi[cols], lo[cols], int(po)).max(axis=1)
+    return e
 
 
 def mixed_errors(got, want, items):

@@ -55,7 +55,7 @@ def test_mixed_orders(sfmm, orc, op, family, pmin, pmax):
         assert all(f == family for f, *_ in info)
         assert any(rt for _, _, rt, _ in info)
     want = orc.translate(op, items, src, dst0.copy())
-    assert_mixed_parity(got, want, items, label=f"op={op} family={family}")
+    assert_mixed_parity(got, want, items, label=f"op={op} family={family}", case=(op, src, dst0))
 
 
 def test_zero_shift_mixed(sfmm, orc):
@@ -70,7 +70,7 @@ def test_zero_shift_mixed(sfmm, orc):
             sel = np.maximum(items["p_in"], items["p_out"]) <= pmax
             got, _ = run(sfmm, op, items[sel], src, dst0, pmax, family)
             want = orc.translate(op, items[sel], src, dst0.copy())
-            assert_mixed_parity(got, want, items[sel], label=f"op={op} family={family}")
+            assert_mixed_parity(got, want, items[sel], label=f"op={op} family={family}", case=(op, src, dst0))
 
 
 def test_graph_replay_matches_execute(sfmm):
@@ -116,7 +116,7 @@ def test_autotune_keeps_results(sfmm, orc):
     plan.execute(s, d)
     torch.cuda.synchronize()
     want = orc.translate(W.M2L, items, src, dst.copy())
-    assert_mixed_parity(d.cpu().numpy(), want, items, label="autotuned")
+    assert_mixed_parity(d.cpu().numpy(), want, items, label="autotuned", case=(W.M2L, src, dst))
     ctx.set_tuning(W.M2L, 10, GENERIC)
     assert ctx.tuning(W.M2L)[10] == GENERIC
     with pytest.raises(sfmm.UnsupportedError):
@@ -139,7 +139,7 @@ def test_mixed_full_size_sample(sfmm, orc):
     sub = items[idx]
     want = orc.translate(W.M2L, sub, src, np.zeros_like(dst))
     # compare only the sampled blocks
-    assert_mixed_parity(got, want, sub, label="sample")
+    assert_mixed_parity(got, want, sub, label="sample", case=(W.M2L, src, None))
     assert np.isfinite(got).all()
     assert plan.launches <= 17
 

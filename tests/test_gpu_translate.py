@@ -67,7 +67,7 @@ def test_golden(sfmm, case):
     print(f"case={case}", flush=True)
     got = run_gpu(sfmm, op, items, src, dst0)
     pi, po = orders(items)
-    assert_parity(got, dst, pi, po, label=case)
+    assert_parity(got, dst, pi, po, label=case, case=(op, items, src, dst0))
 
 
 SQUARE_CASES = ([(LANE, P) for P in range(1, LANE_MAX + 1)] + [(MFMA, P) for P in range(1, MFMA_MAX + 1)]
@@ -80,7 +80,7 @@ def test_golden_generic(sfmm, case):
     op, items, src, dst0, dst = GOLDEN[case]
     got = run_gpu(sfmm, op, items, src, dst0, family=GENERIC)
     pi, po = orders(items)
-    assert_parity(got, dst, pi, po, label=case)
+    assert_parity(got, dst, pi, po, label=case, case=(op, items, src, dst0))
 
 
 @pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
@@ -95,7 +95,7 @@ def test_generic_mixed_orders(sfmm, orc, op, pi, po):
     dst0 = W.random_solids(9, 4, 40, po).reshape(-1)
     got = run_gpu(sfmm, op, items, src, dst0, family=AUTO)
     want = orc.translate(op, items, src, dst0.copy())
-    assert_parity(got, want, pi, po, label=f"{op} {pi}->{po}")
+    assert_parity(got, want, pi, po, label=f"{op} {pi}->{po}", case=(op, items, src, dst0))
 
 
 def physical_batch(orc, op, pi, po, n, seed):
@@ -142,7 +142,7 @@ def test_oracle_parity_square(sfmm, orc, op, family, P):
     print(f"op={op} family={family} P={P}", flush=True)
     got = run_gpu(sfmm, op, items, src, dst0, family=family)
     want = orc.translate(op, items, src, dst0.copy())
-    assert_parity(got, want, P, P, label=f"op={op} family={family} P={P}")
+    assert_parity(got, want, P, P, label=f"op={op} family={family} P={P}", case=(op, items, src, dst0))
 
 
 def test_repeated_targets_apply_in_order(sfmm, orc):
@@ -155,7 +155,7 @@ def test_repeated_targets_apply_in_order(sfmm, orc):
     want = orc.translate(W.M2L, items, src, dst0.copy())
     # Each target sums ~13 contributions whose cancellation is counted in
     # the per-target relative error; a lost or doubled update is O(1).
-    assert_parity(got, want, 10, 10, tol=1e-12)
+    assert_parity(got, want, 10, 10, tol=1e-12, case=(W.M2L, items, src, dst0))
 
 
 @pytest.mark.parametrize("family", [LANE, MFMA])
@@ -168,7 +168,7 @@ def test_zero_shift_and_mixed_lanes(sfmm, orc, family):
         dst0 = W.random_solids(12, 7, 130, 8).reshape(-1)
         got = run_gpu(sfmm, op, items, src, dst0, family=family)
         want = orc.translate(op, items, src, dst0.copy())
-        assert_parity(got, want, 8, 8)
+        assert_parity(got, want, 8, 8, case=(op, items, src, dst0))
 
 
 def test_validation_writes_nothing(sfmm):
@@ -241,4 +241,4 @@ def test_full_size_m2l_p10(sfmm, orc):
     sub["dst"] = np.arange(len(idx), dtype=np.uint64) * np.uint64(110)
     want = orc.translate(W.M2L, sub, src, np.zeros(len(idx) * 110))
     got = a.reshape(n, 110)[idx]
-    assert_parity(got, want, 10, 10, label="sample")
+    assert_parity(got, want, 10, 10, label="sample", case=(W.M2L, sub, src, None))

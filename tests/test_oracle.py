@@ -88,7 +88,7 @@ def test_oracle_matches_golden(orc, case):
     op, items, src, dst0, dst = GOLDEN[case]
     got = orc.translate(op, items, src, dst0.copy())
     pi, po = orders(items)
-    assert_parity(got, dst, pi, po, label=case)
+    assert_parity(got, dst, pi, po, label=case, case=(op, items, src, dst0))
 
 
 @pytest.mark.parametrize("op", [W.M2M, W.M2L, W.L2L])
@@ -98,7 +98,7 @@ def test_oracle_matches_naive(orc, op, pi, po):
     items, src, dst = W.uniform_batch(op, pi, po, 12, 4242 + 31 * pi + po, shift=shift)
     a = orc.translate(op, items, src, dst.copy())
     b = orc.translate(op, items, src, dst.copy(), naive=True)
-    assert_parity(a, b, pi, po, label="naive")
+    assert_parity(a, b, pi, po, label="naive", case=(op, items, src, dst))
 
 
 def test_harmonics_closed_forms(orc):
