@@ -51,6 +51,24 @@ def algorithmic_flops(op, P):
     return 4 * (s2 + s2) + 14 * (t + t) + fz
 
 
+# Host threads of the CPU baseline: one per physical core of the affinity
+# mask, at most the CPU share a one-GPU job gets on the GPU hosts (16 CPUs),
+# unless --cpu-threads says otherwise.
+HOST_SHARE = 16
+
+
+def reference_pool(ref, op, P, items, src, dst, threads, repeats=5):
+    """Median wall time of the compiled reference's batch protocol (pinned
+    worker pool, warm-up on the first 1024 items, `repeats` timed calls over
+    the list; oracle/ref_shim.cpp) and the host description for the line."""
+    info = ref.host_info()
+    if not threads:
+        threads = min(info["physical_cores"], HOST_SHARE)
+    secs, used, pinned = ref.pool_translate(op, P, items, src, dst, threads=threads, repeats=repeats)
+    info.update({"threads": used, "threads_pinned": pinned})
+    return float(np.median(secs)), info
+
+
 def cpu_baseline(op, items, src, gpu_out, P, threads):
     """Time the reference's CPU path (oracle/_ref) on the same batch; fall back
     to the oracle port when the reference library was not built."""
@@ -60,10 +78,17 @@ def cpu_baseline(op, items, src, gpu_out, P, threads):
     dst = np.zeros(n * P * (P + 1))
     if oracle.Reference.available():
         ref = oracle.Reference()
-        secs = ref.translate(opi, P, items, src, dst, threads=threads, repeats=5, chunk=0)
-        kind, cores = "reference", threads
-        note = (f"vorticle/solidfmm 1.5 single-thread {op}_unchecked on {threads} std::threads, targets "
-                "hash-partitioned (multithreaded_translator needs hwloc, absent); whole batch, best of 5")
+        secs, info = reference_pool(ref, opi, P, items, src, dst, threads)
+        # one thread on a slice of the batch: the per-core rate
+        n1 = min(n, 20000)
+        secs1, _ = reference_pool(ref, opi, P, items[:n1], src, np.zeros(n1 * P * (P + 1)), 1, repeats=3)
+        kind, cores = "reference", info["threads"]
+        note = (f"vorticle/solidfmm 1.5, {op}_unchecked on a pool of {cores} worker threads pinned one per "
+                "physical core (the protocol of multithreaded_translator, which needs hwloc, absent here), "
+                "targets split by hash range, 1337-item chunks; warm-up on 1024 items, then the whole "
+                "batch, median of 5")
+        extra = dict(info, median_s=secs, per_thread=n1 / secs1,
+                     projected_all_physical_cores=n1 / secs1 * info["physical_cores"])
         # The reference's AVX-512 lanes 12-15 defect (microkernel_avx512f.cpp:2182)
         # makes its whole-batch output unfit as a checker; check the GPU
         # against the reference in calls of <= 12 items instead.
@@ -77,12 +102,13 @@ def cpu_baseline(op, items, src, gpu_out, P, threads):
         t0 = time.perf_counter()
         orc.translate(opi, items, src, dst)
         secs = time.perf_counter() - t0
-        kind, cores, note = "port", 1, "oracle FP64 restatement, 1 thread, whole batch"
+        kind, cores, note, extra = "port", 1, "oracle FP64 restatement, 1 thread, whole batch", {}
         nchk = min(n, 4096)
         check = dst[: nchk * P * (P + 1)]
     got = gpu_out.reshape(n, -1)[:nchk]
     err = oracle.degree_errors(got, check.reshape(nchk, -1), P)
-    return {"value": n / secs, "unit": "translations/s", "cores": cores, "kind": kind, "sample": note}, err
+    return dict({"value": n / secs, "unit": "translations/s", "cores": cores, "kind": kind, "sample": note},
+                **extra), err
 
 
 def truth_errors(op, items, src, gpu_out, P, nchk=4096):
@@ -216,15 +242,15 @@ def mixed_mode(args):
     }
     if not args.no_cpu_baseline:
         import oracle
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
         if oracle.Reference.available():
             ref = oracle.Reference()
             out = np.zeros_like(dst)
-            secs = ref.translate(W.M2L, args.pmax, items, src, out, threads=threads, repeats=3, chunk=0)
-            line["cpu_baseline"] = {"value": n / secs, "unit": "translations/s", "cores": threads,
-                                    "kind": "reference",
-                                    "sample": f"vorticle/solidfmm 1.5 single-thread m2l_unchecked on {threads} "
-                                              "std::threads (targets hash-partitioned), whole batch, best of 3"}
+            secs, info = reference_pool(ref, W.M2L, args.pmax, items, src, out, args.cpu_threads, repeats=3)
+            line["cpu_baseline"] = dict({"value": n / secs, "unit": "translations/s", "cores": info["threads"],
+                                         "kind": "reference",
+                                         "sample": f"vorticle/solidfmm 1.5 m2l_unchecked on {info['threads']} pinned "
+                                                   "worker threads (multithreaded_translator's protocol), warm-up "
+                                                   "1024 items, whole batch, median of 3"}, **info)
         nchk = 2000
         idx = np.linspace(0, n - 1, nchk).astype(np.int64)
         sub = items[idx]
@@ -452,18 +478,18 @@ def fmm_mode(args):
         import oracle
         if oracle.Reference.available():
             ref = oracle.Reference()
-            threads = args.cpu_threads or min(16, os.cpu_count() or 1)
             mpool = rng.uniform(-1, 1, fmm.pool_len)
             lpool = np.zeros(fmm.pool_len)
             secs = 0.0
             for op, it, _, _, _ in fmm.passes:
                 src = lpool if op == 2 else mpool
                 dst = mpool if op == 0 else lpool
-                secs += ref.translate(op, 20, it, src, dst, threads=threads, repeats=1, chunk=0)
-            cpu = {"value": n_tr / secs, "unit": "translations/s", "cores": threads, "kind": "reference",
-                   "sample": f"vorticle/solidfmm 1.5 single-thread *_unchecked on {threads} std::threads, targets "
-                             "hash-partitioned, the M2M/M2L/L2L passes of one evaluation in order (P2M/L2P not "
-                             f"timed); {secs:.2f} s"}
+                t, info = reference_pool(ref, op, 20, it, src, dst, args.cpu_threads, repeats=1)
+                secs += t
+            cpu = dict({"value": n_tr / secs, "unit": "translations/s", "cores": info["threads"], "kind": "reference",
+                        "sample": f"vorticle/solidfmm 1.5 *_unchecked on {info['threads']} pinned worker threads "
+                                  "(multithreaded_translator's protocol), the M2M/M2L/L2L passes of one evaluation "
+                                  f"in order (P2M/L2P not timed); {secs:.2f} s"}, **info)
     line = {"metric": "far-field FMM evaluations of a Plummer sphere: translations/s over P2M..L2P",
             "value": n_tr / (ms * 1e-3), "unit": "translations/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
@@ -697,8 +723,7 @@ def main():
                      "fp64_tflops": value * fpt / 1e12, "fp64_frac": value * fpt / 78.6e12},
     }
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        base, err = cpu_baseline(op, items, src, gpu_out, P, threads)
+        base, err = cpu_baseline(op, items, src, gpu_out, P, args.cpu_threads)
         line["cpu_baseline"] = base
         per_item = err.max(axis=1)
         line["rel_err_vs_reference"] = {"p50": float(np.percentile(per_item, 50)),

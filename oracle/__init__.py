@@ -164,6 +164,12 @@ class Reference:
                                    ctypes.c_double, _dp]
         L.ref_bounding_sphere.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp]
         L.ref_check_errors.argtypes = [ctypes.c_int] + [ctypes.c_size_t] * 5
+        L.ref_pool_translate_f64.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                             _dp, _dp, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, _dp,
+                                             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.ref_host_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_char_p,
+                                    ctypes.c_size_t, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
 
     @staticmethod
     def available(path=REF_LIB):
@@ -190,6 +196,33 @@ class Reference:
         if rc != 0:
             raise RuntimeError(f"reference translate failed ({rc})")
         return secs.value
+
+    def pool_translate(self, op, P_op, items, src, dst, threads=0, warmup=1024, repeats=5):
+        """The reference's batch protocol on a pinned worker pool (one thread
+        per physical core of the affinity mask, or `threads`): warm-up on the
+        first `warmup` items, then `repeats` timed calls over the list.
+        Returns (seconds per call, workers, workers pinned)."""
+        items = np.ascontiguousarray(items)
+        secs = np.zeros(repeats)
+        used, pinned = ctypes.c_int(), ctypes.c_int()
+        rc = self.lib.ref_pool_translate_f64(op, P_op, items.ctypes.data, len(items), _ptr(src), _ptr(dst),
+                                             threads, warmup, repeats, _ptr(secs), ctypes.byref(used),
+                                             ctypes.byref(pinned))
+        if rc != 0:
+            raise RuntimeError(f"reference pool translate failed ({rc})")
+        return secs, used.value, pinned.value
+
+    def host_info(self):
+        """cpu_model, nproc, cpus in the affinity mask, physical cores in it,
+        and the microkernel the reference selects here (with its tile)."""
+        model = ctypes.create_string_buffer(256)
+        kern = ctypes.create_string_buffer(64)
+        ints = [ctypes.c_int() for _ in range(5)]
+        self.lib.ref_host_info(model, 256, ctypes.byref(ints[0]), ctypes.byref(ints[1]), ctypes.byref(ints[2]),
+                               kern, 64, ctypes.byref(ints[3]), ctypes.byref(ints[4]))
+        return {"cpu_model": model.value.decode(), "nproc": ints[0].value, "affinity_cpus": ints[1].value,
+                "physical_cores": ints[2].value, "microkernel": kern.value.decode(),
+                "microkernel_tile": [ints[3].value, ints[4].value]}
 
     def swap_entry(self, n, m, l):
         v = ctypes.c_double()

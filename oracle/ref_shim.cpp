@@ -19,10 +19,18 @@
 // chunk <= 12 keeps every item in lanes 0-11, where results are correct.
 //
 // multithreaded_translator is not built: it needs hwloc, which this image
-// lacks.  ref_translate_threads() runs the reference's single-thread entry
-// point on N std::threads instead, each owning the targets of one hash range
-// (the partitioning multithreaded_translator.cpp:298-319 uses), with its own
-// buffer_handle.
+// lacks.  translate() with nthreads > 1 runs the reference's single-thread
+// entry point on N std::threads instead, each owning the targets of one hash
+// range, with its own buffer_handle.  ref_pool_translate_f64() follows the
+// batch protocol of multithreaded_translator (multithreaded_translator.cpp:
+// 200-330, 375-654) and of its benchmark (tests/benchmark_multithreaded_
+// translator.cpp:64-88) more closely: persistent workers, one pinned to each
+// physical core of the process's affinity mask (sched_setaffinity in place
+// of hwloc), created before any timing and released per call by barriers;
+// every worker scans the whole list and runs the items whose target hash
+// falls in its equal share of the hash range, in chunks of 1337 through
+// the *_unchecked entry points; a warm-up call on the first 1024 items, then
+// timed calls over the full list.
 #include <solidfmm/sphere.hpp>
 #include <solidfmm/handles.hpp>
 #include <solidfmm/harmonics.hpp>
@@ -30,13 +38,28 @@
 #include <solidfmm/swap_matrix.hpp>
 #include <solidfmm/translations.hpp>
 
+#include <solidfmm/microkernel.hpp>
+#include <solidfmm/microkernel_avx.hpp>
+#include <solidfmm/microkernel_avx512f.hpp>
+
+#include <atomic>
+#include <barrier>
 #include <chrono>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <exception>
+#include <fstream>
 #include <map>
+#include <memory>
+#include <set>
+#include <string>
 #include <thread>
 #include <vector>
+
+#include <pthread.h>
+#include <sched.h>
+#include <unistd.h>
 
 namespace {
 
@@ -199,6 +222,183 @@ int translate(int op, size_t P, const shim_item *items, size_t n, const real *sr
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Pinned worker pool over the reference's single-thread entry points.
+// ---------------------------------------------------------------------------
+
+// One CPU per physical core of the affinity mask (the first CPU of each
+// (package, core) pair), from sysfs topology; every CPU in the mask when the
+// topology is unreadable.
+std::vector<int> physical_core_cpus(int *nmask)
+{
+    cpu_set_t mask;
+    CPU_ZERO(&mask);
+    sched_getaffinity(0, sizeof(mask), &mask);
+    std::set<std::pair<int, int>> seen;
+    std::vector<int> cpus;
+    int count = 0;
+    for (int c = 0; c < CPU_SETSIZE; ++c) {
+        if (!CPU_ISSET(c, &mask)) continue;
+        ++count;
+        char path[128];
+        int core = -1, pkg = -1;
+        std::snprintf(path, sizeof path, "/sys/devices/system/cpu/cpu%d/topology/core_id", c);
+        std::ifstream(path) >> core;
+        std::snprintf(path, sizeof path, "/sys/devices/system/cpu/cpu%d/topology/physical_package_id", c);
+        std::ifstream(path) >> pkg;
+        if (core < 0 || pkg < 0) { cpus.push_back(c); continue; }
+        if (seen.insert({pkg, core}).second) cpus.push_back(c);
+    }
+    if (nmask) *nmask = count;
+    return cpus;
+}
+
+// Equal shares of the hash range, as schedule_equally_among_threads.
+inline int owner(uint64_t h, int nthreads)
+{
+    return static_cast<int>((static_cast<unsigned __int128>(h) * static_cast<unsigned>(nthreads)) >> 64);
+}
+
+template <typename real>
+struct pool {
+    using item = solidfmm::translation_info<real>;
+    const solidfmm::operator_handle<real> &oph;
+    int op = 1, n = 0;
+    const item *begin = nullptr, *end = nullptr;
+    bool die = false;
+    std::barrier<> start, done;
+    std::vector<std::thread> th;
+    std::vector<int> cpus;
+    int pinned = 0;
+
+    pool(const solidfmm::operator_handle<real> &o, std::vector<int> c)
+        : oph(o), n(static_cast<int>(c.size())), start(n + 1), done(n + 1), cpus(std::move(c))
+    {
+        std::barrier<> ready(n + 1);
+        std::atomic<int> ok{0};
+        for (int t = 0; t < n; ++t)
+            th.emplace_back([this, t, &ready, &ok] {
+                cpu_set_t one;
+                CPU_ZERO(&one);
+                CPU_SET(cpus[t], &one);
+                if (pthread_setaffinity_np(pthread_self(), sizeof(one), &one) == 0) ok.fetch_add(1);
+                solidfmm::buffer_handle<real> bh = oph.make_buffer();
+                std::vector<item> buf(1337);
+                ready.arrive_and_wait();
+                while (true) {
+                    start.arrive_and_wait();
+                    if (die) return;
+                    size_t nb = 0;
+                    for (const item *i = begin; i != end; ++i) {
+                        if (owner(target_hash(reinterpret_cast<uintptr_t>(i->target)), n) != t) continue;
+                        buf[nb++] = *i;
+                        if (nb == buf.size()) {
+                            call<real>(op, oph, bh, buf.data(), buf.data() + nb, false);
+                            nb = 0;
+                        }
+                    }
+                    if (nb) call<real>(op, oph, bh, buf.data(), buf.data() + nb, false);
+                    done.arrive_and_wait();
+                }
+            });
+        ready.arrive_and_wait();
+        pinned = ok.load();
+    }
+
+    double run(int o, const item *b, const item *e)
+    {
+        op = o;
+        begin = b;
+        end = e;
+        auto t0 = std::chrono::steady_clock::now();
+        start.arrive_and_wait();
+        done.arrive_and_wait();
+        auto t1 = std::chrono::steady_clock::now();
+        return std::chrono::duration<double>(t1 - t0).count();
+    }
+
+    ~pool()
+    {
+        die = true;
+        start.arrive_and_wait();
+        for (auto &x : th) x.join();
+    }
+};
+
+const char *microkernel_name()
+{
+    std::unique_ptr<solidfmm::microkernel<double>> k(solidfmm::get_microkernel<double>());
+    if (dynamic_cast<solidfmm::microkernel_double_avx512f *>(k.get())) return "microkernel_double_avx512f";
+    if (dynamic_cast<solidfmm::microkernel_double_avx *>(k.get())) return "microkernel_double_avx";
+    return "microkernel_double_generic";
+}
+
+}  // namespace
+
+extern "C" {
+
+// Host description for the CPU baseline: cpu model (from /proc/cpuinfo),
+// online CPUs, CPUs in the affinity mask, physical cores in the mask, the
+// microkernel the reference selects on this host, and its tile.
+int ref_host_info(char *model, size_t model_len, int *nproc, int *nmask, int *physical_cores, char *kernel,
+                  size_t kernel_len, int *rows, int *cols)
+{
+    std::string m = "unknown";
+    std::ifstream f("/proc/cpuinfo");
+    for (std::string line; std::getline(f, line);)
+        if (line.rfind("model name", 0) == 0) {
+            auto p = line.find(':');
+            if (p != std::string::npos) m = line.substr(p + 2);
+            break;
+        }
+    std::snprintf(model, model_len, "%s", m.c_str());
+    *nproc = static_cast<int>(sysconf(_SC_NPROCESSORS_ONLN));
+    *physical_cores = static_cast<int>(physical_core_cpus(nmask).size());
+    std::snprintf(kernel, kernel_len, "%s", microkernel_name());
+    std::unique_ptr<solidfmm::microkernel<double>> k(solidfmm::get_microkernel<double>());
+    *rows = static_cast<int>(k->rows);
+    *cols = static_cast<int>(k->cols);
+    return 0;
+}
+
+// Times the list on the pinned pool: `threads` workers (0: one per physical
+// core of the affinity mask), a warm-up call on the first `warmup` items,
+// then `repeats` timed calls over the full list, each on zeroed targets;
+// seconds[r] receives each timed call's wall time.  The targets hold one
+// application of the list at return (lanes 12-15 of its 16-wide batches
+// carry the AVX-512 defect noted above; timing is unaffected).  *used
+// receives the worker count, *pinned how many took their CPU.
+int ref_pool_translate_f64(int op, size_t P, const void *items, size_t n, const double *src_pool,
+                           double *dst_pool, int threads, size_t warmup, int repeats, double *seconds, int *used,
+                           int *pinned)
+{
+    try {
+        if (op < 0 || op > 2 || repeats < 1) return -1;
+        staged<double> s;
+        stage(s, static_cast<const shim_item *>(items), n, src_pool, dst_pool);
+        solidfmm::operator_handle<double> oph(P);
+        std::vector<int> cpus = physical_core_cpus(nullptr);
+        if (threads > 0 && static_cast<size_t>(threads) < cpus.size()) cpus.resize(threads);
+        pool<double> pl(oph, cpus);
+        *used = pl.n;
+        *pinned = pl.pinned;
+        const auto *b = s.list.data();
+        pl.run(op, b, b + std::min(n, warmup));
+        for (int r = 0; r < repeats; ++r) {
+            for (auto &d : s.dsts) d.zeros();
+            seconds[r] = pl.run(op, b, b + n);
+        }
+        unstage(s, dst_pool);
+        return 0;
+    } catch (const std::exception &) {
+        return -2;
+    }
+}
+
+}  // extern "C"
+
+namespace {
 }  // namespace
 
 extern "C" {

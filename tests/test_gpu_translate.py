@@ -242,3 +242,33 @@ def test_full_size_m2l_p10(sfmm, orc):
     want = orc.translate(W.M2L, sub, src, np.zeros(len(idx) * 110))
     got = a.reshape(n, 110)[idx]
     assert_parity(got, want, 10, 10, label="sample", case=(W.M2L, sub, src, None))
+
+
+GOLDEN32 = load_golden("golden_f32.npz")
+
+
+@pytest.mark.parametrize("case", sorted(GOLDEN32))
+def test_golden_float_path(sfmm, orc, case):
+    """The reference's float outputs (tests/golden/golden_f32.npz).  Float
+    solids take the drop-in's float path: widened to FP64, translated on the
+    GPU, rounded to float once.  Against the FP64 result on the same float
+    inputs, every value is within one float rounding plus FP64 noise, and
+    the per-degree errors stay within the reference float path's own."""
+    op, items, src32, dst0_32, dst32 = GOLDEN32[case]
+    pi, po = orders(items)
+    src, dst0 = src32.astype(np.float64), dst0_32.astype(np.float64)
+    exact = orc.translate(op, items, src, dst0.copy())
+    got = run_gpu(sfmm, op, items, src, dst0).astype(np.float32).astype(np.float64)
+    scale = np.zeros_like(exact)
+    P = po
+    blk = P * (P + 1)
+    for b in range(len(exact) // blk):
+        for n in range(P):
+            sl = slice(b * blk + n * (n + 1), b * blk + (n + 1) * (n + 2))
+            scale[sl] = np.abs(exact[sl]).max()
+    bound = 2.0 ** -24 * np.abs(exact) + 1e-12 * scale
+    assert np.all(np.abs(got - exact) <= bound), float((np.abs(got - exact) / bound).max())
+    e = oracle.degree_errors(got, exact, P).max(axis=1)
+    e_ref = oracle.degree_errors(dst32.astype(np.float64), exact, P).max(axis=1)
+    print(f"  {case}: float path max {e.max():.2e}, reference float max {e_ref.max():.2e}", flush=True)
+    assert np.median(e) <= np.median(e_ref) and e.max() <= e_ref.max()
