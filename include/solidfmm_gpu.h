@@ -206,6 +206,27 @@ int sfmm_set_tuning(sfmm_context *ctx, int op, uint32_t order, int family);
 int sfmm_launch(sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, const sfmm_item *dev_items,
                 size_t n, const double *src_pool, double *dst_pool, void *stream);
 
+/* Allocation-free translation of a list over device pools.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It runs what sfmm_plan_create + sfmm_plan_execute would run
+ * on the same list (same validation, kernels and results), but the plan's
+ * device tables and, when targets repeat, its per-item scratch blocks live
+ * in a static device array of the library, SFMM_NOALLOC_SCRATCH_BYTES
+ * (64 MiB) per GPU, set up by sfmm_create and held until the process ends;
+ * the call itself allocates no device memory.  A list that needs more
+ * scratch than that is SFMM_ERR_LOGIC and nothing is enqueued (a list of
+ * distinct targets needs 48 bytes per item plus 256-byte alignment; one
+ * with repeated targets about P(P+1) 8 + 64 bytes per item more).  The
+ * tables are copied from host memory on `stream` before the kernels.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (other streams, other threads, other contexts) race and must be
+ * serialised by the caller.  The array keeps the last call's contents;
+ * nothing in it needs to start at zero. */
+#define SFMM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int sfmm_translate_noalloc(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, const double *src_pool,
+                           size_t src_len, double *dst_pool, size_t dst_len, void *stream);
+
 /* Host-pool convenience: copies the pools to the device, runs the list and
  * copies the target pool back (blocking).  The drop-in C++ API and the
  * classic single-thread entry points use this path. */

@@ -85,6 +85,8 @@ SIGNATURES = {
     "sfmm_plan_execute": (_c.c_int, [_vp, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp]),
     "sfmm_launch": (_c.c_int, [_vp, _c.c_int, _c.c_uint32, _c.c_uint32, _vp, _c.c_size_t, _vp, _vp, _vp]),
     "sfmm_translate_host": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t]),
+    "sfmm_translate_noalloc": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t,
+                                          _vp]),
     "sfmm_swap_block": (_c.c_int, [_c.c_int, _c.c_int, _c.POINTER(_c.c_double)]),
     "sfmm_synchronize": (_c.c_int, [_vp, _vp]),
     "sfmm_plan_launch_info": (_c.c_int, [_vp, _c.c_size_t, _c.POINTER(_c.c_int), _c.POINTER(_c.c_uint32),

@@ -429,6 +429,46 @@ hipError_t enqueue(sfmm_plan *plan, const double *src, double *dst, hipStream_t 
 // allocation happens inside one.
 bool tables_ready() { return sfmm::mfma::device_table() && sfmm::mfma::device_factorials(); }
 
+// Device scratch of the allocation-free entry point (sfmm_translate_noalloc):
+// the plan's tables (items, reduction targets and items) and, when targets
+// repeat, its per-item scratch blocks.  One copy per GPU, held until the
+// process ends.  Host code never uses the array's name as an address: the
+// per-device address comes from hipGetSymbolAddress at set-up.
+constexpr size_t kNoAllocBytes = SFMM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+struct ScratchAddresses {
+    std::mutex mu;
+    std::vector<unsigned char *> per_device;
+};
+
+ScratchAddresses &scratch_addresses()
+{
+    static ScratchAddresses s;
+    return s;
+}
+
+// Once per device, with that device current.
+hipError_t scratch_setup(int device)
+{
+    ScratchAddresses &st = scratch_addresses();
+    std::lock_guard<std::mutex> lock(st.mu);
+    if (st.per_device.size() <= size_t(device)) st.per_device.resize(device + 1, nullptr);
+    if (st.per_device[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    st.per_device[device] = static_cast<unsigned char *>(ptr);
+    return hipSuccess;
+}
+
+unsigned char *scratch_address(int device)
+{
+    ScratchAddresses &st = scratch_addresses();
+    std::lock_guard<std::mutex> lock(st.mu);
+    return size_t(device) < st.per_device.size() ? st.per_device[device] : nullptr;
+}
+
 }  // namespace
 
 extern "C" {
@@ -445,6 +485,15 @@ int sfmm_create(sfmm_context **out, int device, size_t P, int precision)
         return SFMM_ERR_OVERFLOW;
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return SFMM_ERR_HIP;
+    // Set-up on the device (its current device restored afterwards): the
+    // swap and factorial tables, and the address of the allocation-free
+    // entry point's scratch array.
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess) return SFMM_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return SFMM_ERR_HIP;
+    const bool ready = tables_ready() && scratch_setup(device) == hipSuccess;
+    (void)hipSetDevice(prev);
+    if (!ready) return SFMM_ERR_HIP;
     sfmm_context *ctx = new (std::nothrow) sfmm_context;
     if (!ctx) return SFMM_ERR_ALLOC;
     ctx->device = device;
@@ -678,6 +727,42 @@ int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, d
     e = enqueue(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
     return SFMM_OK;
+}
+
+int sfmm_translate_noalloc(sfmm_context *ctx, int op, const sfmm_item *host_items, size_t n, const double *src_pool,
+                           size_t src_len, double *dst_pool, size_t dst_len, void *stream)
+{
+    if (!ctx) return SFMM_ERR_LOGIC;
+    sfmm_plan *plan = nullptr;
+    int rc = sfmm_plan_prepare(ctx, op, host_items, n, &plan);  // host memory only
+    if (rc != SFMM_OK) return rc;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (plan->dev_bytes > kNoAllocBytes) {
+        rc = fail(ctx, SFMM_ERR_LOGIC,
+                  "solidfmm: the list needs " + std::to_string(plan->dev_bytes) +
+                      " bytes of device scratch; sfmm_translate_noalloc has " + std::to_string(kNoAllocBytes) +
+                      " (SFMM_NOALLOC_SCRATCH_BYTES)");
+    } else if (plan->n_items && (!src_pool || !dst_pool)) {
+        rc = fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pool");
+    } else if (src_len < plan->src_extent || dst_len < plan->dst_extent) {
+        rc = fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
+    } else if (plan->n_items) {
+        unsigned char *ptr = scratch_address(ctx->device);
+        hipError_t e = ptr ? hipSetDevice(ctx->device) : hipErrorNotInitialized;
+        // the tables come from the plan's pageable host memory: the runtime
+        // stages such a copy before returning, so the plan may go at once
+        if (e == hipSuccess && !plan->blob.empty())
+            e = hipMemcpyAsync(ptr, plan->blob.data(), plan->blob.size(), hipMemcpyHostToDevice,
+                               static_cast<hipStream_t>(stream));
+        if (e == hipSuccess) {
+            plan->d_mem = ptr;
+            e = enqueue(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream));
+        }
+        if (e != hipSuccess) rc = hip_fail(ctx, e, "sfmm_translate_noalloc");
+    }
+    plan->d_mem = nullptr;  // the scratch array is the library's, not the plan's
+    delete plan;
+    return rc;
 }
 
 int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len)
