@@ -53,6 +53,7 @@ struct sfmm_plan {
     struct Launch {
         sfmm::KernelChoice k;
         size_t offset, count;
+        uint32_t round = 0;  // launches of one round write pairwise distinct targets
     };
     sfmm_context *ctx = nullptr;
     int op = 0;
@@ -311,7 +312,8 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         const Key &key = keys[k];
         if (k == 0 || keys[k - 1].round != key.round || keys[k - 1].family != key.family ||
             keys[k - 1].pb != key.pb) {
-            plan.launches.push_back({{key.family, key.pb, false, ctx->precision, !plan.scratch}, ordered.size(), 0});
+            plan.launches.push_back(
+                {{key.family, key.pb, false, ctx->precision, !plan.scratch}, ordered.size(), 0, key.round});
             plan.launches.back().k.seg =
                 runs && (key.family == sfmm::kLaneKernels || key.family == sfmm::kMfmaKernels);
             prev = nullptr;
@@ -416,6 +418,76 @@ hipError_t enqueue(sfmm_plan *plan, const double *src, double *dst, hipStream_t 
     for (const auto &L : plan->launches) {
         hipError_t e = sfmm::launch_kernel(plan->op, L.k, plan->d_items() + L.offset, L.count, src, out, stream);
         if (e != hipSuccess) return e;
+    }
+    if (plan->scratch)
+        return sfmm::launch_reduce(reinterpret_cast<const sfmm::RedTarget *>(plan->d_mem + plan->off_targets),
+                                   plan->n_targets,
+                                   reinterpret_cast<const sfmm::RedItem *>(plan->d_mem + plan->off_red_items), out,
+                                   dst, stream);
+    return hipSuccess;
+}
+
+// Relative cost of a launch, to balance the branches of a captured round:
+// items times the algorithmic bytes of its order bound.
+double launch_cost(const sfmm_plan::Launch &L)
+{
+    const double p = double(L.k.pk);
+    return double(L.count) * (3.0 * p * (p + 1.0) + 3.0);
+}
+
+// As enqueue, with the launches of each round spread over `stream` and the
+// extra streams `aux` (forked from and joined back into `stream` by
+// events): they write pairwise distinct targets, so they may run at once
+// and one launch's tail overlaps the next's start.  Rounds stay in order;
+// the reduction follows the last round.  Used under graph capture, where
+// the branches become parallel graph nodes.
+hipError_t enqueue_branched(sfmm_plan *plan, const double *src, double *dst, hipStream_t stream,
+                            const std::vector<hipStream_t> &aux, const std::vector<hipEvent_t> &ev)
+{
+    double *out = plan->scratch ? reinterpret_cast<double *>(plan->d_mem + plan->off_scratch) : dst;
+    const size_t nb = aux.size() + 1;
+    for (size_t i = 0; i < plan->launches.size();) {
+        size_t j = i;
+        while (j < plan->launches.size() && plan->launches[j].round == plan->launches[i].round) ++j;
+        hipError_t e = hipSuccess;
+        if (j - i == 1 || nb == 1) {
+            for (size_t k = i; k < j && e == hipSuccess; ++k) {
+                const auto &L = plan->launches[k];
+                e = sfmm::launch_kernel(plan->op, L.k, plan->d_items() + L.offset, L.count, src, out, stream);
+            }
+        } else {
+            // heaviest launches first, each on the least loaded branch
+            std::vector<size_t> order;
+            for (size_t k = i; k < j; ++k) order.push_back(k);
+            std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+                return launch_cost(plan->launches[a]) > launch_cost(plan->launches[b]);
+            });
+            std::vector<double> load(nb, 0.0);
+            std::vector<std::vector<size_t>> branch(nb);
+            for (size_t k : order) {
+                const size_t b = size_t(std::min_element(load.begin(), load.end()) - load.begin());
+                branch[b].push_back(k);
+                load[b] += launch_cost(plan->launches[k]);
+            }
+            e = hipEventRecord(ev[0], stream);
+            for (size_t b = 1; b < nb && e == hipSuccess; ++b)
+                if (!branch[b].empty()) e = hipStreamWaitEvent(aux[b - 1], ev[0], 0);
+            for (size_t b = 0; b < nb && e == hipSuccess; ++b) {
+                hipStream_t s = b ? aux[b - 1] : stream;
+                for (size_t k : branch[b]) {
+                    const auto &L = plan->launches[k];
+                    e = sfmm::launch_kernel(plan->op, L.k, plan->d_items() + L.offset, L.count, src, out, s);
+                    if (e != hipSuccess) break;
+                }
+            }
+            for (size_t b = 1; b < nb && e == hipSuccess; ++b) {
+                if (branch[b].empty()) continue;
+                e = hipEventRecord(ev[b], aux[b - 1]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(stream, ev[b], 0);
+            }
+        }
+        if (e != hipSuccess) return e;
+        i = j;
     }
     if (plan->scratch)
         return sfmm::launch_reduce(reinterpret_cast<const sfmm::RedTarget *>(plan->d_mem + plan->off_targets),
@@ -778,19 +850,36 @@ int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, d
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     if (!tables_ready()) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: device tables");
     drop_graph(plan);
+    // the capture stream plus kBranches - 1 forked ones (default priority)
+    constexpr size_t kBranches = 4;
     hipStream_t s = nullptr;
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> ev;
     e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamCreate");
+    for (size_t b = 1; b < kBranches && e == hipSuccess; ++b) {
+        hipStream_t a = nullptr;
+        e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+        if (e == hipSuccess) aux.push_back(a);
+    }
+    for (size_t b = 0; b < kBranches && e == hipSuccess; ++b) {
+        hipEvent_t x = nullptr;
+        e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        if (e == hipSuccess) ev.push_back(x);
+    }
     hipGraph_t graph = nullptr;
-    e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
     if (e == hipSuccess) {
-        hipError_t le = enqueue(plan, src_pool, dst_pool, s);
-        e = hipStreamEndCapture(s, &graph);
-        if (le != hipSuccess) e = le;
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
+        if (e == hipSuccess) {
+            hipError_t le = enqueue_branched(plan, src_pool, dst_pool, s, aux, ev);
+            e = hipStreamEndCapture(s, &graph);
+            if (le != hipSuccess) e = le;
+        }
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
     if (graph) (void)hipGraphDestroy(graph);
-    (void)hipStreamDestroy(s);
+    for (hipEvent_t x : ev) (void)hipEventDestroy(x);
+    for (hipStream_t a : aux) (void)hipStreamDestroy(a);
+    if (s) (void)hipStreamDestroy(s);
     if (e != hipSuccess) {
         plan->exec = nullptr;
         return hip_fail(ctx, e, "graph capture");

@@ -74,6 +74,20 @@ uint32_t persistent_grid(const void *kernel, int block, size_t work)
     return resident;
 }
 
+// Items per wave of a lane launch of n items: 64, unless the last round of
+// resident waves would be partly empty -- then the items are spread evenly
+// over whole rounds (1e5 items on 1,024 resident waves: 2,041 waves of 49
+// instead of 1,563 of 64, two full rounds either way).
+uint32_t lane_items_per_wave(const void *kernel, size_t n)
+{
+    const size_t slots = persistent_grid(kernel, 64, size_t(1) << 40);
+    const size_t waves = (n + 63) / 64;
+    if (slots == 0 || waves <= slots) return 64;
+    const size_t rounds = (waves + slots - 1) / slots;
+    const size_t ipw = (n + rounds * slots - 1) / (rounds * slots);
+    return uint32_t(ipw < 1 ? 1 : ipw > 64 ? 64 : ipw);
+}
+
 namespace {
 
 template <bool RT>
@@ -111,9 +125,11 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
         const lane::Table &t = (k.rt || runs) ? lane::rt_table() : lane::fixed_table();
         lane::kernel_fn f = t.f[op][k.pk - 1];
         const uint32_t block = 64;
-        const uint32_t grid = static_cast<uint32_t>((n + block - 1) / block);
+        uint32_t ipw = lane_items_per_wave(reinterpret_cast<const void *>(f), n);
+        if (runs) ipw = 64;  // the plan lays runs out in 64-item waves
+        const uint32_t grid = static_cast<uint32_t>((n + ipw - 1) / ipw);
         hipLaunchKernelGGL(f, dim3(grid), dim3(block), 0, stream, src, dst, items, static_cast<uint32_t>(n),
-                           runs ? 2 : acc);
+                           runs ? 2 : acc, ipw);
         return hipGetLastError();
     }
     case kMfmaKernels: {
