@@ -136,6 +136,32 @@ def truth_errors(op, items, src, gpu_out, P, nchk=4096):
     return rep
 
 
+def arith_mode(args):
+    """Issue rates of the arithmetic the precision modes trade against
+    (sfmm_probe_arith), and what they imply for a double-word FP32 path."""
+    import ctypes
+    import solidfmm_amd
+    lib = solidfmm_amd.load()
+    names = ["fp64_fma_valu", "fp32_fma_valu", "dw32_fma_valu", "fp64_mfma_16x16x4", "fp32_mfma_16x16x4"]
+    rates = {}
+    for k, name in enumerate(names):
+        v = ctypes.c_double()
+        rc = lib.sfmm_probe_arith(0, k, ctypes.byref(v))
+        if rc != 0:
+            raise RuntimeError(f"sfmm_probe_arith({k}) failed ({rc})")
+        rates[name] = v.value
+    line = {"mode": "arith", "unit": "flop/s", "rates": rates,
+            "ratios": {"fp32_valu_over_fp64_valu": rates["fp32_fma_valu"] / rates["fp64_fma_valu"],
+                       "dw32_valu_over_fp64_valu": rates["dw32_fma_valu"] / rates["fp64_fma_valu"],
+                       "fp32_mfma_over_fp64_mfma": rates["fp32_mfma_16x16x4"] / rates["fp64_mfma_16x16x4"],
+                       # hi/lo split products need 3 FP32 products per FP64 one
+                       # (and still accumulate in FP32); exact 12-bit slices of
+                       # 48-bit operands need 10
+                       "split3_mfma_over_fp64_mfma": rates["fp32_mfma_16x16x4"] / 3 / rates["fp64_mfma_16x16x4"],
+                       "slices10_mfma_over_fp64_mfma": rates["fp32_mfma_16x16x4"] / 10 / rates["fp64_mfma_16x16x4"]}}
+    print(json.dumps(line), flush=True)
+
+
 def copy_mode(args):
     """Plain device-to-device copy of known size: reads and writes
     args.copy_mib MiB per launch."""
@@ -582,7 +608,8 @@ def main():
     ap.add_argument("--latency-reps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "serving", "fmm", "host", "copy"], default="translate",
+    ap.add_argument("--mode", choices=["translate", "mixed", "precision", "serving", "fmm", "host", "copy", "arith"],
+                    default="translate",
                     help="mixed: the mixed-order M2L workload (orders --pmin..--pmax, default 500,000 items) "
                          "as one captured HIP graph; copy: time a plain device copy of --copy-mib MiB (HBM "
                          "reference and the calibration kernel for the FETCH_SIZE / WRITE_SIZE counters)")
@@ -611,6 +638,8 @@ def main():
         sys.exit(2)
     if args.mode == "copy":
         return copy_mode(args)
+    if args.mode == "arith":
+        return arith_mode(args)
     if args.mode == "mixed":
         return mixed_mode(args)
     if args.mode == "precision":

@@ -98,7 +98,7 @@ size_t sfmm_max_order(void);
  * operator_handle<real>(P) (handles.hpp:40, handles.cpp:42-45): P == 0 is
  * SFMM_ERR_LOGIC, P above sfmm_max_order() (24 for the reduced modes) is
  * SFMM_ERR_OVERFLOW.  Reduced-precision contexts run square M2L of orders
- * <= 12 (FP32) or <= 10 (split) in a one-lane-per-translation kernel and
+ * <= 16 (FP32) or <= 10 (split) in a one-lane-per-translation kernel and
  * everything else in a one-workgroup-per-translation kernel (no MFMA
  * family, no autotuning). */
 int sfmm_create(sfmm_context **ctx, int device, size_t P, int precision);
@@ -168,7 +168,11 @@ int sfmm_plan_launch_info(const sfmm_plan *plan, size_t i, int *family, uint32_t
 
 /* Run a plan: target pool += T(source pool), asynchronously on `stream`.
  * Pools are device pointers of src_len / dst_len reals; a pool shorter than
- * the plan's extent is SFMM_ERR_LOGIC and nothing is launched.  Replaces
+ * the plan's extent is SFMM_ERR_LOGIC and nothing is launched.  Launches
+ * that write pairwise distinct targets (the order buckets of one round) run
+ * on `stream` and up to three streams of the context forked from it and
+ * joined back by events, so one bucket's tail overlaps the next's start;
+ * everything enqueued on `stream` afterwards sees the whole plan done.  Replaces
  * multithreaded_translator<real>::m2m/m2l/l2l (translations.hpp:75-81) for
  * device-resident pools. */
 int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, double *dst_pool, size_t dst_len,
@@ -176,7 +180,8 @@ int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, d
 
 /* Capture the plan's launches on these pools into a HIP graph kept with the
  * plan (replacing an earlier capture); sfmm_plan_replay then enqueues the
- * whole plan as one graph launch on any stream.  The pools are bound at
+ * whole plan as one graph launch on any stream.  The buckets of a round
+ * become parallel branches of the graph, as in sfmm_plan_execute.  The pools are bound at
  * capture: capture again to run on other pools.  No counterpart in the
  * reference (a launch-overhead tool for lists that are replayed, such as
  * the same interaction lists every time step). */
@@ -278,6 +283,14 @@ int sfmm_p2m(sfmm_cellset *set, const double *dev_points, size_t n_points, doubl
  * 3 D n_points. */
 int sfmm_l2p(sfmm_cellset *set, const double *dev_points, size_t n_points, const double *dev_pool, size_t pool_len,
              double *dev_phi, double *dev_grad, void *stream);
+
+/* Diagnostics with no counterpart in the reference: the device's issue
+ * rate (flop/s, a multiply-add counted as 2) of the arithmetic the
+ * precision modes trade against, operands in registers, eight waves per
+ * SIMD.  kind 0: FP64 FMA (VALU), 1: FP32 FMA (VALU, not packed), 2: the
+ * double-word FP32 multiply-add of SFMM_SPLIT32, 3: FP64 MFMA 16x16x4,
+ * 4: FP32 MFMA 16x16x4.  Allocates and frees a small output buffer. */
+int sfmm_probe_arith(int device, int kind, double *flops_per_s);
 
 /* HIP device of a context (-1 for NULL). */
 int sfmm_device(const sfmm_context *ctx);

@@ -88,6 +88,7 @@ SIGNATURES = {
     "sfmm_translate_noalloc": (_c.c_int, [_vp, _c.c_int, _vp, _c.c_size_t, _vp, _c.c_size_t, _vp, _c.c_size_t,
                                           _vp]),
     "sfmm_swap_block": (_c.c_int, [_c.c_int, _c.c_int, _c.POINTER(_c.c_double)]),
+    "sfmm_probe_arith": (_c.c_int, [_c.c_int, _c.c_int, _c.POINTER(_c.c_double)]),
     "sfmm_synchronize": (_c.c_int, [_vp, _vp]),
     "sfmm_plan_launch_info": (_c.c_int, [_vp, _c.c_size_t, _c.POINTER(_c.c_int), _c.POINTER(_c.c_uint32),
                                          _c.POINTER(_c.c_int), _c.POINTER(_c.c_size_t)]),

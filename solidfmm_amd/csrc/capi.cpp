@@ -47,6 +47,11 @@ struct sfmm_context {
     // staging of the blocking host-pool call (sfmm_translate_host)
     std::mutex host_mu;
     sfmm_staging *host_staging = nullptr;
+    // streams forked from the caller's stream to run the launches of one
+    // round side by side (sfmm_plan_execute), and their fork/join events;
+    // created on first use
+    std::vector<hipStream_t> branch_streams;
+    std::vector<hipEvent_t> branch_events;
 };
 
 struct sfmm_plan {
@@ -497,6 +502,35 @@ hipError_t enqueue_branched(sfmm_plan *plan, const double *src, double *dst, hip
     return hipSuccess;
 }
 
+// Branches used by sfmm_plan_execute and sfmm_plan_capture: the caller's
+// stream and three forked ones (default priority).
+constexpr size_t kBranches = 4;
+
+bool has_parallel_round(const sfmm_plan *plan)
+{
+    for (size_t k = 1; k < plan->launches.size(); ++k)
+        if (plan->launches[k].round == plan->launches[k - 1].round) return true;
+    return false;
+}
+
+// The context's branch streams and events, created on first use (with the
+// context's device current and its lock held).
+hipError_t ensure_branches(sfmm_context *ctx)
+{
+    hipError_t e = hipSuccess;
+    while (e == hipSuccess && ctx->branch_streams.size() + 1 < kBranches) {
+        hipStream_t a = nullptr;
+        e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+        if (e == hipSuccess) ctx->branch_streams.push_back(a);
+    }
+    while (e == hipSuccess && ctx->branch_events.size() < kBranches) {
+        hipEvent_t x = nullptr;
+        e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+        if (e == hipSuccess) ctx->branch_events.push_back(x);
+    }
+    return e;
+}
+
 // Device tables the kernels read, created before any capture so that no
 // allocation happens inside one.
 bool tables_ready() { return sfmm::mfma::device_table() && sfmm::mfma::device_factorials(); }
@@ -578,6 +612,10 @@ int sfmm_create(sfmm_context **out, int device, size_t P, int precision)
 void sfmm_destroy(sfmm_context *ctx)
 {
     if (ctx && ctx->host_staging) sfmm_staging_destroy(ctx->host_staging);
+    if (ctx) {
+        for (hipEvent_t x : ctx->branch_events) (void)hipEventDestroy(x);
+        for (hipStream_t a : ctx->branch_streams) (void)hipStreamDestroy(a);
+    }
     delete ctx;
 }
 
@@ -796,7 +834,34 @@ int sfmm_plan_execute(sfmm_plan *plan, const double *src_pool, size_t src_len, d
         return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    e = enqueue(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream));
+    if (has_parallel_round(plan)) {
+        e = ensure_branches(ctx);
+        if (e != hipSuccess) return hip_fail(ctx, e, "branch streams");
+        e = enqueue_branched(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream), ctx->branch_streams,
+                             ctx->branch_events);
+    } else {
+        e = enqueue(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream));
+    }
+    if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
+    return SFMM_OK;
+}
+
+// One chain of launches on the caller's stream, as sfmm_plan_execute was
+// before branching: the request loop (server.cpp) overlaps requests on its
+// own slot streams, and branches shared by all slots would couple them.
+__attribute__((visibility("hidden"))) int sfmm_plan_execute_chain(sfmm_plan *plan, const double *src_pool,
+                                                                 size_t src_len, double *dst_pool, size_t dst_len,
+                                                                 void *stream)
+{
+    if (!plan) return SFMM_ERR_LOGIC;
+    sfmm_context *ctx = plan->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    if (plan->n_items && (!src_pool || !dst_pool)) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: null pool");
+    if (plan->n_items && !plan->d_mem) return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: plan not bound to device memory");
+    if (src_len < plan->src_extent || dst_len < plan->dst_extent)
+        return fail(ctx, SFMM_ERR_LOGIC, "solidfmm: pool shorter than the plan's extent");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e == hipSuccess) e = enqueue(plan, src_pool, dst_pool, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(ctx, e, "kernel launch");
     return SFMM_OK;
 }
@@ -851,7 +916,6 @@ int sfmm_plan_capture(sfmm_plan *plan, const double *src_pool, size_t src_len, d
     if (!tables_ready()) return fail(ctx, SFMM_ERR_ALLOC, "solidfmm: device tables");
     drop_graph(plan);
     // the capture stream plus kBranches - 1 forked ones (default priority)
-    constexpr size_t kBranches = 4;
     hipStream_t s = nullptr;
     std::vector<hipStream_t> aux;
     std::vector<hipEvent_t> ev;

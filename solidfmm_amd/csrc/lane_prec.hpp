@@ -1,4 +1,5 @@
-// Reduced-precision M2L, one lane per translation, square orders 1..12:
+// Reduced-precision M2L, one lane per translation, square orders 1..16
+// (FP32; double-word FP32 up to 10):
 // the register-resident mapping of lane_kernel.hpp with the bulk arithmetic
 // (rotations, swaps, z-translation) in an arithmetic policy of prec.hpp.
 //

@@ -44,6 +44,11 @@
 
 #include "../../include/solidfmm_gpu.h"
 
+// Library-internal (capi.cpp): sfmm_plan_execute without branch streams.
+extern "C" __attribute__((visibility("hidden"))) int sfmm_plan_execute_chain(sfmm_plan *plan, const double *src_pool,
+                                                                           size_t src_len, double *dst_pool,
+                                                                           size_t dst_len, void *stream);
+
 namespace {
 
 constexpr size_t kAlign = 256;
@@ -233,7 +238,8 @@ struct sfmm_server {
         const int rc = r.staged ? sfmm_plan_bind_pinned(r.plan, mem, r.arena_len, host_pool + r.host_off, s)
                                 : sfmm_plan_bind(r.plan, mem, r.arena_len, s);
         if (rc != SFMM_OK) return false;
-        if (sfmm_plan_execute(r.plan, pool[sp], pool_len[sp], pool[dp], pool_len[dp], s) != SFMM_OK) return false;
+        if (sfmm_plan_execute_chain(r.plan, pool[sp], pool_len[sp], pool[dp], pool_len[dp], s) != SFMM_OK)
+            return false;
         return hipEventRecord(r.ev_end, s) == hipSuccess;
     }
 

@@ -56,7 +56,7 @@ uint32_t persistent_grid(const void *kernel, int block, size_t work);
 // Reduced-precision kernels (arithmetic policies of prec.hpp): a lane
 // kernel for square M2L up to kPrecLaneMax and a generic kernel for every
 // operator and order up to kPrecMaxOrder.
-inline constexpr uint32_t kPrecLaneMax = 12;    // FP32
+inline constexpr uint32_t kPrecLaneMax = 16;    // FP32
 inline constexpr uint32_t kPrecLaneMaxDW = 10;  // double-word FP32
 inline constexpr uint32_t kPrecMaxOrder = 24;
 hipError_t launch_prec_lane_m2l(int precision, uint32_t P, const sfmm_item *items, size_t n, const double *src,

@@ -73,10 +73,11 @@ def test_precision_against_fp64(sfmm, orc, op, P):
             assert np.percentile(e, 99) <= 1e-11
 
 
-@pytest.mark.parametrize("precision", [FP32, SPLIT])
-def test_lane_and_generic_precision_kernels_agree(sfmm, orc, precision):
-    P = 8
-    items, src, dst = W.uniform_batch(W.M2L, P, P, 800, 0xBEAD)
+@pytest.mark.parametrize("precision,P", [(FP32, 8), (FP32, 13), (FP32, 16), (SPLIT, 8), (SPLIT, 10)])
+def test_lane_and_generic_precision_kernels_agree(sfmm, orc, precision, P):
+    """The one-lane-per-translation kernels (FP32 to order 16, double-word
+    to 10) against the one-workgroup kernel and the FP64 error."""
+    items, src, dst = W.uniform_batch(W.M2L, P, P, 800, 0xBEAD + P)
     a, pa = run(sfmm, W.M2L, items, src, dst, P, precision, family=1)
     b, pb = run(sfmm, W.M2L, items, src, dst, P, precision, family=3)
     assert pa.launch_info()[0][0] == 1 and pb.launch_info()[0][0] == 3
@@ -93,11 +94,18 @@ def test_precision_mixed_orders(sfmm, orc, precision):
     dst0 = W.uniform(0x779, precision, len(dst), -1.0, 1.0)
     got, _ = run(sfmm, W.M2L, items, src, dst0, 24, precision)
     want = orc.translate(W.M2L, items, src, dst0.copy())
+    assert np.isfinite(got).all()
     if precision == SPLIT:
         # double-word: the FP64 tolerances, 100x
         assert_mixed_parity(got, want, items, scale=100.0, label="split mixed")
-    else:
-        assert np.isfinite(got).all()
+    # per item against the FP64 kernel's error on the same items, at twice
+    # the ratio of unit roundoffs (50th and 99th percentiles)
+    ref, _ = run(sfmm, W.M2L, items, src, dst0, 24, FP64)
+    e64 = oracle.mixed_errors(ref, want, items)
+    e = oracle.mixed_errors(got, want, items)
+    for q in (50, 99):
+        bound = RATIO[precision] * max(np.percentile(e64, q), 1e-16)
+        assert np.percentile(e, q) <= bound, (precision, q, np.percentile(e, q), bound)
 
 
 def test_precision_context_limits(sfmm):
