@@ -15,7 +15,8 @@
 // immediate operand (zero entries are skipped), and the rotated triangle of
 // coefficients lives in registers; nothing but the source, the target and
 // the item touches memory.  The swap contractions run on the FP64 VALU: on
-// MI355X the FP64 MFMA peak equals the FP64 VALU peak (78.6 TFLOP/s each)
+// MI355X the FP64 MFMA peak equals the FP64 VALU peak on paper (78.6 TFLOP/s
+// each; measured, FP64 FMA sustains 58 and v_mfma_f64_16x16x4f64 47)
 // and the folded swap blocks are half zeros, which a VALU contraction skips
 // and an MFMA tile cannot.
 //
