@@ -1,8 +1,11 @@
 // Reduced-precision translation kernel for any orders (up to the precision
-// limit) and any mix of (p_in, p_out) within one launch: the generic
-// kernel's mapping (one 64-thread workgroup per translation, rotated
-// triangle in LDS) with the bulk arithmetic in an arithmetic policy of
-// prec.hpp (FP32, or double-word FP32).
+// limit) and any mix of (p_in, p_out) within one launch: a group of 16 lanes
+// per translation, four translations per wave (64-thread workgroups), the
+// rotated triangle and the working vectors in the group's own LDS region,
+// lane j of a group handling the orders m = j, j + 16 of each degree; the
+// bulk arithmetic in an arithmetic policy of prec.hpp (FP32, or
+// double-word FP32).  A group's lanes exchange values only through LDS
+// within one wave, so no workgroup barrier is needed past the set-up.
 //
 // Scaled basis (as in lane_prec.hip): order m of degree n is carried scaled
 // by a power of two 2^q(n,m), with 2^e(n,m) ~ sqrt((n-m)!(n+m)!),
@@ -34,6 +37,17 @@ namespace sfmm {
 namespace precgen {
 
 constexpr int kThreads = 64;
+constexpr int kGroup = 16;                    // lanes per translation
+constexpr int kGroups = kThreads / kGroup;    // translations per wave
+
+// Orders LDS writes and reads of one wave's lanes (they run in program
+// order on the wave; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ __forceinline__ int dense_base_rt(int n)
 {
@@ -50,33 +64,54 @@ __device__ __forceinline__ int enorm_rt(const double *__restrict__ fac, int n, i
 
 // out = S_n in (tr: D^-1 S_n^T D in) for the real and imaginary blocks, in
 // the scaled basis: entry (m, l) times 2^(e[m] - e[l]) (M-type) or
-// 2^(e[l] - e[m]) (tr, L-type); e holds e(n, .).
+// 2^(e[l] - e[m]) (tr, L-type); e holds e(n, .).  Lane j of a group
+// computes the rows m = j, j + 16, ...; the real block is zero unless
+// l + m + n is even and the imaginary block unless it is odd
+// (operator_data.cpp:47-147), so each row visits one parity of l per part
+// (skipping exact zeros leaves every sum unchanged).
 template <class Pol>
 __device__ void swap_p(const double *__restrict__ table, int n, bool tr, const int *e, const typename Pol::T *in_r,
-                       const typename Pol::T *in_i, typename Pol::T *out_r, typename Pol::T *out_i)
+                       const typename Pol::T *in_i, typename Pol::T *out_r, typename Pol::T *out_i, int j)
 {
     using T = typename Pol::T;
     const int st = (n + 1) | 1;
     const double *sr = table + dense_base_rt(n);
     const double *si = sr + (n + 1) * st;
-    for (int m = threadIdx.x; m <= n; m += kThreads) {
+    for (int m = j; m <= n; m += kGroup) {
         T ar = Pol::zero(), ai = Pol::zero();
-        for (int l = 0; l <= n; ++l) {
-            double cr, ci;
+        const int lr = (m + n) & 1;  // first l of the real part's parity
+        for (int l = lr; l <= n; l += 2) {
+            double cr;
             if (!tr) {
                 cr = ldexp(sr[m * st + l], e[m] - e[l]);
-                ci = ldexp(si[m * st + l], e[m] - e[l]);
             } else {
                 const double f = (l == 0 ? 1.0 : 2.0) * (m == 0 ? 1.0 : 0.5);
                 cr = ldexp(f * sr[l * st + m], e[l] - e[m]);
-                ci = ldexp(f * si[l * st + m], e[l] - e[m]);
             }
             if (cr != 0.0) ar = Pol::fma(Pol::from(cr), in_r[l], ar);
+        }
+        for (int l = lr ^ 1; l <= n; l += 2) {
+            double ci;
+            if (!tr) {
+                ci = ldexp(si[m * st + l], e[m] - e[l]);
+            } else {
+                const double f = (l == 0 ? 1.0 : 2.0) * (m == 0 ? 1.0 : 0.5);
+                ci = ldexp(f * si[l * st + m], e[l] - e[m]);
+            }
             if (ci != 0.0) ai = Pol::fma(Pol::from(ci), in_i[l], ai);
         }
         out_r[m] = ar;
         out_i[m] = ai;
     }
+}
+
+// LDS of a workgroup: the exponent triangle (16-byte aligned), then per
+// group the triangle (re, im), eight vectors of pc values of T (sized for
+// 8-byte T) and two of pc doubles.
+__host__ __device__ constexpr size_t eb_bytes(int pc) { return (sizeof(int) * size_t(pc) * (pc + 1) / 2 + 15) / 16 * 16; }
+__host__ __device__ constexpr size_t group_bytes(int pc)
+{
+    return 8 * (size_t(pc) * (pc + 1) + 8 * size_t(pc)) + 2 * sizeof(double) * size_t(pc);
 }
 
 template <class Pol>
@@ -88,7 +123,13 @@ __global__ void __launch_bounds__(kThreads)
     using T = typename Pol::T;
     extern __shared__ double sh_raw[];
     const int tri = pc * (pc + 1) / 2;
-    T *tre = reinterpret_cast<T *>(sh_raw);
+    const int tid = threadIdx.x;
+    const int j = tid % kGroup;  // lane within the group
+    // e(n, m) for the triangle, shared by the groups; then one region per
+    // group (group_bytes)
+    int *eb = reinterpret_cast<int *>(sh_raw);
+    unsigned char *gbase = reinterpret_cast<unsigned char *>(sh_raw) + eb_bytes(pc) + (tid / kGroup) * group_bytes(pc);
+    T *tre = reinterpret_cast<T *>(gbase);
     T *tim = tre + tri;
     T *var = tim + tri;
     T *vai = var + pc;
@@ -98,10 +139,8 @@ __global__ void __launch_bounds__(kThreads)
     T *sa = ca + pc;
     T *cb = sa + pc;
     T *sb = cb + pc;
-    double *fs = reinterpret_cast<double *>(sb + pc);  // forward scale per degree
-    double *bs = fs + pc;                               // backward scale per degree
-    int *eb = reinterpret_cast<int *>(bs + pc);         // e(n, m), triangle
-    const int tid = threadIdx.x;
+    double *fs = reinterpret_cast<double *>(gbase + 8 * (2 * tri + 8 * pc));  // forward scale per degree
+    double *bs = fs + pc;                                                    // backward scale per degree
     for (int n = 0; n < pc; ++n)
         for (int m = tid; m <= n; m += kThreads) eb[n * (n + 1) / 2 + m] = enorm_rt(fac, n, m);
     __syncthreads();
@@ -111,7 +150,7 @@ __global__ void __launch_bounds__(kThreads)
         return ltype ? e[0] - e[m] : e[m] - e[n];
     };
 
-    for (uint32_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+    for (uint32_t item = blockIdx.x * kGroups + tid / kGroup; item < n_items; item += gridDim.x * kGroups) {
         const sfmm_item it = items[item];
         const int pi = int(it.p_in), po = int(it.p_out);
         if (pi == 0 || po == 0) continue;
@@ -120,7 +159,7 @@ __global__ void __launch_bounds__(kThreads)
         double x = it.x, y = it.y, z = it.z;
         const int pend = pi < po ? pi : po;
         if (op != SFMM_M2L && x * x + y * y + z * z == 0.0) {  // translations.cpp:539-547, 596-604
-            for (int i = tid; i < pend * (pend + 1); i += kThreads) D[i] = (acc ? D[i] : 0.0) + S[i];
+            for (int i = j; i < pend * (pend + 1); i += kGroup) D[i] = (acc ? D[i] : 0.0) + S[i];
             continue;
         }
         if (op == SFMM_M2M) {
@@ -129,7 +168,7 @@ __global__ void __launch_bounds__(kThreads)
             z = -z;
         }
         const int pm = pi > po ? pi : po;
-        if (tid == 0) {
+        if (j == 0) {
             // Euler data and powers in FP64 (microkernel_generic.cpp:36-97).
             const double rho = sqrt(x * x + y * y);
             const double r = sqrt(x * x + y * y + z * z);
@@ -163,35 +202,35 @@ __global__ void __launch_bounds__(kThreads)
                 rm *= rinv;
             }
         }
-        __syncthreads();
+        wave_sync();
 
         // Forward transforms: M-type for M2M / M2L, L-type for L2L.
         const bool ltype_fwd = (op == SFMM_L2L);
         for (int n = 0; n < pi; ++n) {
             const double scale = fs[n];
             const int *en = eb + n * (n + 1) / 2;
-            for (int m = tid; m <= n; m += kThreads) {
+            for (int m = j; m <= n; m += kGroup) {
                 const int q = qside(ltype_fwd, n, m);
                 const T a = Pol::from(ldexp(scale * S[n * (n + 1) + m], q));
                 const T b = Pol::from(ldexp(scale * S[(n + 1) * (n + 1) + m], q));
                 var[m] = Pol::sub(Pol::mul(ca[m], a), Pol::mul(sa[m], b));
                 vai[m] = Pol::add(Pol::mul(sa[m], a), Pol::mul(ca[m], b));
             }
-            __syncthreads();
-            swap_p<Pol>(table, n, ltype_fwd, en, var, vai, vbr, vbi);
-            __syncthreads();
-            for (int m = tid; m <= n; m += kThreads) {
+            wave_sync();
+            swap_p<Pol>(table, n, ltype_fwd, en, var, vai, vbr, vbi, j);
+            wave_sync();
+            for (int m = j; m <= n; m += kGroup) {
                 const T a = vbr[m], b = vbi[m];
                 var[m] = Pol::sub(Pol::mul(cb[m], a), Pol::mul(sb[m], b));
                 vai[m] = Pol::add(Pol::mul(sb[m], a), Pol::mul(cb[m], b));
             }
-            __syncthreads();
-            swap_p<Pol>(table, n, ltype_fwd, en, var, vai, vbr, vbi);
-            for (int m = tid; m <= n; m += kThreads) {
+            wave_sync();
+            swap_p<Pol>(table, n, ltype_fwd, en, var, vai, vbr, vbi, j);
+            for (int m = j; m <= n; m += kGroup) {
                 tre[n * (n + 1) / 2 + m] = vbr[m];
                 tim[n * (n + 1) / 2 + m] = vbi[m];
             }
-            __syncthreads();
+            wave_sync();
         }
 
         // z-translation and backward transforms: L-type for M2L / L2L,
@@ -201,7 +240,7 @@ __global__ void __launch_bounds__(kThreads)
         const int mlim = (op == SFMM_L2L) ? pend : pi;
         for (int N = 0; N < nend; ++N) {
             const int *eN = eb + N * (N + 1) / 2;
-            for (int m = tid; m <= N; m += kThreads) {
+            for (int m = j; m <= N; m += kGroup) {
                 T ar = Pol::zero(), ai = Pol::zero();
                 const int qN = qside(ltype_bwd, N, m);
                 if (m < mlim) {
@@ -235,18 +274,18 @@ __global__ void __launch_bounds__(kThreads)
                 var[m] = ar;
                 vai[m] = ai;
             }
-            __syncthreads();
-            swap_p<Pol>(table, N, ltype_bwd, eN, var, vai, vbr, vbi);
-            __syncthreads();
-            for (int m = tid; m <= N; m += kThreads) {
+            wave_sync();
+            swap_p<Pol>(table, N, ltype_bwd, eN, var, vai, vbr, vbi, j);
+            wave_sync();
+            for (int m = j; m <= N; m += kGroup) {
                 const T a = vbr[m], b = vbi[m];
                 var[m] = Pol::add(Pol::mul(cb[m], a), Pol::mul(sb[m], b));
                 vai[m] = Pol::sub(Pol::mul(cb[m], b), Pol::mul(sb[m], a));
             }
-            __syncthreads();
-            swap_p<Pol>(table, N, ltype_bwd, eN, var, vai, vbr, vbi);
+            wave_sync();
+            swap_p<Pol>(table, N, ltype_bwd, eN, var, vai, vbr, vbi, j);
             const double scale = bs[N];
-            for (int m = tid; m <= N; m += kThreads) {
+            for (int m = j; m <= N; m += kGroup) {
                 const T a = vbr[m], b = vbi[m];
                 const T re = Pol::add(Pol::mul(ca[m], a), Pol::mul(sa[m], b));
                 const T im = Pol::sub(Pol::mul(ca[m], b), Pol::mul(sa[m], a));
@@ -256,17 +295,12 @@ __global__ void __launch_bounds__(kThreads)
                 D[N * (N + 1) + m] = (acc ? D[N * (N + 1) + m] : 0.0) + vr;
                 D[(N + 1) * (N + 1) + m] = (acc ? D[(N + 1) * (N + 1) + m] : 0.0) + vi;
             }
-            __syncthreads();
+            wave_sync();
         }
     }
 }
 
-size_t lds_bytes(int pc)
-{
-    const size_t tri = size_t(pc) * (pc + 1) / 2;
-    // T is 4 or 8 bytes; size for 8
-    return 8 * (2 * tri + 4 * size_t(pc) + 4 * size_t(pc)) + sizeof(double) * 2 * size_t(pc) + sizeof(int) * tri;
-}
+size_t lds_bytes(int pc) { return eb_bytes(pc) + kGroups * group_bytes(pc); }
 
 template <class Pol>
 hipError_t launch(int op, uint32_t pc, const sfmm_item *items, size_t n, const double *src, double *dst,
@@ -276,7 +310,8 @@ hipError_t launch(int op, uint32_t pc, const sfmm_item *items, size_t n, const d
     const double *fac = mfma::device_factorials();
     if (!table || !fac) return hipErrorOutOfMemory;
     const size_t lds = lds_bytes(int(pc));
-    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_translate_prec<Pol>), kThreads, n);
+    const uint32_t grid = persistent_grid(reinterpret_cast<const void *>(&k_translate_prec<Pol>), kThreads,
+                                          (n + kGroups - 1) / kGroups);
     hipLaunchKernelGGL(k_translate_prec<Pol>, dim3(grid), dim3(kThreads), lds, stream, op, src, dst, items,
                        static_cast<uint32_t>(n), table, fac, fac + 2 * mfma::kMaxOrder, int(pc), acc);
     return hipGetLastError();
