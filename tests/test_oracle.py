@@ -257,3 +257,23 @@ def test_reference_error_types(ref):
     assert ref.check_errors(1, 10, 10, 2, 10, 1) == 1
     assert ref.check_errors(1, 0, 0, 1, 0, 1) == 1
     assert ref.check_errors(1, 87, 10, 1, 10, 1) == 3
+
+
+@pytest.mark.reference
+def test_reference_pool_protocol(ref):
+    """The CPU baseline's run of the reference (oracle/ref_shim.cpp,
+    ref_pool_translate_f64): workers pinned one per core, a warm-up call,
+    timed calls on zeroed targets.  With at most 8 items per call it stays
+    clear of the AVX-512 lane defect, so its targets equal the reference's
+    single-thread result; the host description names the microkernel."""
+    items, src, dst = W.uniform_batch(W.M2L, 10, 10, 8, 77)
+    want = dst.copy()
+    ref.translate(W.M2L, 10, items, src, want)
+    got = dst.copy()
+    secs, used, pinned = ref.pool_translate(W.M2L, 10, items, src, got, threads=2, warmup=4, repeats=3)
+    assert used == 2 and pinned == 2
+    assert len(secs) == 3 and np.all(secs > 0)
+    assert np.array_equal(got, want)
+    info = ref.host_info()
+    assert info["physical_cores"] >= 1 and info["nproc"] >= info["physical_cores"]
+    assert info["microkernel"].startswith("microkernel_double_")
