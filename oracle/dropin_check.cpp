@@ -42,6 +42,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #ifndef SFMM_WITH_MT
@@ -222,7 +223,15 @@ void forms(const char *tname, size_t P, size_t dim, uint64_t seed)
     const operator_data<real> &od = oph;
     threadlocal_buffer<real> &tb = bh;
 #if SFMM_WITH_MT
+    // the rest of the class's surface (translations.hpp:47-87): the
+    // topology constructor, both scheduling policies, move construction and
+    // move assignment; the object that results runs the mt forms below
+    multithreaded_translator<real> first(P, nullptr, nullptr, nullptr);
+    first.schedule_by_measuring_throughput();
+    multithreaded_translator<real> moved(std::move(first));
     multithreaded_translator<real> mt(P);
+    mt = std::move(moved);
+    mt.schedule_equally_among_threads();
 #endif
     for (int op = 0; op < 3; ++op) {
         std::string base = std::string(tname) + "_" + op_name(op) + "_d" + std::to_string(dim);
