@@ -65,7 +65,9 @@ def reference_pool(ref, op, P, items, src, dst, threads, repeats=5):
     if not threads:
         threads = min(info["physical_cores"], HOST_SHARE)
     secs, used, pinned = ref.pool_translate(op, P, items, src, dst, threads=threads, repeats=repeats)
-    info.update({"threads": used, "threads_pinned": pinned})
+    info.update({"threads": used, "threads_pinned": pinned,
+                 "batch_latency_ms": {"p50": float(np.median(secs)) * 1e3, "max": float(np.max(secs)) * 1e3,
+                                      "min": float(np.min(secs)) * 1e3, "calls": len(secs)}})
     return float(np.median(secs)), info
 
 
