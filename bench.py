@@ -92,7 +92,7 @@ def cpu_baseline(op, items, src, gpu_out, P, threads):
         extra = dict(info, median_s=secs, per_thread=n1 / secs1,
                      projected_all_physical_cores=n1 / secs1 * info["physical_cores"])
         # The reference's AVX-512 lanes 12-15 defect (microkernel_avx512f.cpp:2182)
-        # makes its whole-batch output unfit as a checker; check the GPU
+        # makes its whole-batch output unfit for comparison; check the GPU
         # against the reference in calls of <= 12 items instead.
         nchk = min(n, 4096)
         check = np.zeros(nchk * P * (P + 1))
@@ -636,7 +636,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md, not implemented", file=sys.stderr)
+              "sharding over several GPUs is not implemented", file=sys.stderr)
         sys.exit(2)
     if args.mode == "copy":
         return copy_mode(args)

@@ -5,7 +5,7 @@
  * hand-written HIP kernels on one AMD Instinct MI355X (gfx950).  The C++
  * drop-in headers under include/solidfmm/ (the reference's public API) are
  * implemented on top of these entry points; other languages bind them
- * directly (see INTEGRATION.md).
+ * directly.
  *
  * Reference interfaces replaced (file:line in vorticle/solidfmm 1.5):
  *   solid<real> coefficient layout        solidfmm/solid.hpp:57-60
@@ -73,7 +73,7 @@ enum {
     SFMM_ERR_OVERFLOW = -3,    /* std::overflow_error: order too large for the type */
     SFMM_ERR_ALLOC = -4,       /* std::bad_alloc: host or device allocation failed */
     SFMM_ERR_HIP = -5,         /* std::runtime_error: a HIP runtime call failed */
-    SFMM_ERR_UNSUPPORTED = -6  /* valid request this build has no kernel for */
+    SFMM_ERR_UNSUPPORTED = -6  /* valid request this library has no kernel for */
 };
 
 /* One translation.  48 bytes, naturally aligned.  Replaces
@@ -90,7 +90,7 @@ typedef struct sfmm_item {
 typedef struct sfmm_context sfmm_context;
 typedef struct sfmm_plan sfmm_plan;
 
-/* Highest order this build supports (FP64): 86, the reference's limit. */
+/* Highest order the library supports (FP64): 86, the reference's limit. */
 size_t sfmm_max_order(void);
 
 /* Context on HIP device `device` for orders up to P, with the arithmetic

@@ -1,4 +1,4 @@
-"""ctypes bindings of the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+"""ctypes bindings of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
 
 ``Oracle`` wraps liboracle.so, the FP64 restatement of the reference
 algorithm (oracle.c).  ``Truth`` wraps libtruth.so, the same operators in

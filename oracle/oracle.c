@@ -2,7 +2,7 @@
  * CPU oracle for the translation operators (M2M, M2L, L2L) on solid
  * harmonics -- TEST INFRASTRUCTURE ONLY.
  *
- * This file is the checker for the HIP product path.  Only the test-suite,
+ * This file is the oracle of the HIP product path.  Only the test-suite,
  * the smoke check and the CPU-baseline leg of the benchmark may load it.  The
  * product library never links, calls or falls back to it.
  *

@@ -2,7 +2,7 @@
  * Extended-precision truth for the translation operators (M2M, M2L, L2L)
  * -- TEST INFRASTRUCTURE ONLY.
  *
- * This file is a checker.  Only the test-suite, the smoke check and the
+ * This file is part of the oracle.  Only the test-suite, the smoke check and the
  * CPU-baseline leg of the benchmark may load it; the product library never
  * links, calls or falls back to it.
  *
@@ -12,8 +12,8 @@
  *
  *   method 0, naive: the O(P^4) double sums of doc/solidfmm.tex:1404-1433
  *     (M2M sum over l = -k..k) over the solid harmonics R and S, evaluated by
- *     the recurrences of harmonics.cpp:31-128 (Appendix A of the survey:
- *     diagonal, then down each column m).
+ *     the recurrences of harmonics.cpp:31-128 (the diagonal first, then
+ *     down each column m).
  *   method 1, rotation: the O(P^3) algorithm of translations.cpp:242-471 --
  *     Euler data by repeated complex products (microkernel_generic.cpp:36-97),
  *     z rotations (:99-140), the folded real/imaginary swap blocks

@@ -125,7 +125,7 @@ class HipError(SolidfmmError):
 
 
 class UnsupportedError(SolidfmmError, NotImplementedError):
-    """A valid request this build has no kernel for."""
+    """A valid request this library has no kernel for."""
 
 
 _ERRORS = {

@@ -46,7 +46,7 @@ bool family_covers(int family, uint32_t pb);
 // The family kAuto picks for items whose larger order is pb.
 int default_family(uint32_t pb);
 
-// True when this build has a kernel for (op, p_in, p_out) in `family`.
+// True when the library has a kernel for (op, p_in, p_out) in `family`.
 bool have_kernel(int op, uint32_t p_in, uint32_t p_out, int family = kAuto);
 
 // Workgroups for a persistent kernel: as many as can be resident on the

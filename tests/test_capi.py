@@ -67,7 +67,7 @@ def test_library_reports_order_limit(lib):
 
 
 def test_product_path_has_no_oracle_dependency():
-    """The product package must not import or load the checkers."""
+    """The product package must not import or load the oracle."""
     pkg = os.path.join(ROOT, "solidfmm_amd")
     for dirpath, _, files in os.walk(pkg):
         for f in files:

@@ -63,7 +63,7 @@ def _kernel_available(items):
 def test_golden(sfmm, case):
     op, items, src, dst0, dst = GOLDEN[case]
     if not _kernel_available(items):
-        pytest.skip("orders without a kernel in this build")
+        pytest.skip("orders without a kernel in this library")
     print(f"case={case}", flush=True)
     got = run_gpu(sfmm, op, items, src, dst0)
     pi, po = orders(items)
