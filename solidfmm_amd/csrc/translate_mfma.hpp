@@ -514,36 +514,41 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             static_for_down<0, PO>([&](auto Nc) { out_degree(Nc, rp[decltype(Nc)::value]); });
         }
     } else if constexpr (OP == SFMM_M2M) {
+        // Stage s runs the forward transform of source degree s beside the
+        // z-translation and backward transform of output degree s - 1
+        // (which needs source degrees K <= s - 1 only): two independent
+        // chains of MFMAs per stage.
         double sc = rinv;  // forward r^-(n+1)
         double sb = r;     // backward r^(N+1) (translations.cpp:129)
         load_degree(ic<0>{}, pr, pim);
-        fetch_old(ic<0>{}, orr, oi);
-        static_for<0, PM>([&](auto nc) {
-            constexpr int n = decltype(nc)::value;
+        static_for<0, PM + 1>([&](auto stc) {
+            constexpr int st = decltype(stc)::value;
+            constexpr int n = st;      // forward degree
+            constexpr int N = st - 1;  // output degree
             double xr[NSL], xi[NSL], cr[NSL], ci[NSL];
             if constexpr (n < PI) {
                 take(xr, pr);
                 take(xi, pim);
                 if constexpr (n + 1 < PI) load_degree(ic<n + 1>{}, pr, pim);
             }
-            if constexpr (n < PO) {
+            if constexpr (N >= 0 && N < PO) {
                 take(cr, orr);
                 take(ci, oi);
-                if constexpr (n + 1 < PO) fetch_old(ic<n + 1>{}, orr, oi);
             }
+            if constexpr (N + 1 < PO) fetch_old(ic<N + 1>{}, orr, oi);
             sched_fence();
             if constexpr (n < PI) {
                 if (!RT || uint32_t(n) < pin_w) {
                     transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
-                    keep_degree(nc, xr, xi);
+                    keep_degree(ic<n>{}, xr, xi);
                 } else {
-                    clear_degree(nc);
+                    clear_degree(ic<n>{});
                 }
                 sc *= rinv;
             }
-            if constexpr (n < PO) {
-                constexpr int N = n;
+            if constexpr (N >= 0 && N < PO) {
                 if (!RT || uint32_t(N) < pout_w) {
+                    double yr[NSL], yi[NSL];
                     static_for<0, NSL>([&](auto scc) {
                         constexpr int s = decltype(scc)::value;
                         double ar = 0.0, ai = 0.0;
@@ -558,11 +563,11 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                             ar = (q + 4 * s > N) ? 0.0 : ar;
                             ai = (q + 4 * s > N) ? 0.0 : ai;
                         }
-                        xr[s] = ar;
-                        xi[s] = ai;
+                        yr[s] = ar;
+                        yi[s] = ai;
                     });
-                    transform_bwd<N, false>(tab, lane, R, sb, xr, xi);
-                    store_sum(ic<N>{}, cr, ci, xr, xi);
+                    transform_bwd<N, false>(tab, lane, R, sb, yr, yi);
+                    store_sum(ic<N>{}, cr, ci, yr, yi);
                 }
                 sb *= r;
             }
@@ -582,31 +587,36 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             constexpr int n = decltype(nc)::value;
             bs[n] = bs[n - 1] * rinv;
         });
+        // Stage n runs the forward transform of source degree n beside the
+        // z-translation and backward transform of output degree n + 1
+        // (which needs source degrees K >= n + 1 only).
         load_degree(ic<PI - 1>{}, pr, pim);
-        if constexpr (PE - 1 == PI - 1) fetch_old(ic<PE - 1>{}, orr, oi);
-        static_for_down<0, PI>([&](auto nc) {
-            constexpr int n = decltype(nc)::value;
+        static_for_down<-1, PI>([&](auto stc) {
+            constexpr int n = decltype(stc)::value;  // forward degree
+            constexpr int N = n + 1;                 // output degree
             double xr[NSL], xi[NSL], cr[NSL], ci[NSL];
-            take(xr, pr);
-            take(xi, pim);
-            if constexpr (n > 0) load_degree(ic<n - 1>{}, pr, pim);
-            if constexpr (n < PE) {
+            if constexpr (n >= 0) {
+                take(xr, pr);
+                take(xi, pim);
+                if constexpr (n > 0) load_degree(ic<n - 1>{}, pr, pim);
+            }
+            if constexpr (N < PE) {
                 take(cr, orr);
                 take(ci, oi);
-                if constexpr (n > 0) fetch_old(ic<n - 1>{}, orr, oi);
-            } else if constexpr (n == PE) {
-                fetch_old(ic<PE - 1>{}, orr, oi);
             }
+            if constexpr (n >= 0 && n < PE) fetch_old(ic<n>{}, orr, oi);
             sched_fence();
-            if (!RT || uint32_t(n) < pin_w) {
-                transform_fwd<n, true>(tab, lane, R, fs[n], xr, xi);
-                keep_degree(nc, xr, xi);
-            } else {
-                clear_degree(nc);
+            if constexpr (n >= 0) {
+                if (!RT || uint32_t(n) < pin_w) {
+                    transform_fwd<n, true>(tab, lane, R, fs[n], xr, xi);
+                    keep_degree(ic<n>{}, xr, xi);
+                } else {
+                    clear_degree(ic<n>{});
+                }
             }
-            if constexpr (n < PE) {
-                constexpr int N = n;
+            if constexpr (N < PE) {
                 if (!RT || uint32_t(N) < pout_w) {
+                    double yr[NSL], yi[NSL];
                     static_for<0, NSL>([&](auto scc) {
                         constexpr int s = decltype(scc)::value;
                         double ar = 0.0, ai = 0.0;
@@ -620,11 +630,11 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                             ar = (q + 4 * s > N) ? 0.0 : ar;
                             ai = (q + 4 * s > N) ? 0.0 : ai;
                         }
-                        xr[s] = ar;
-                        xi[s] = ai;
+                        yr[s] = ar;
+                        yi[s] = ai;
                     });
-                    transform_bwd<N, true>(tab, lane, R, bs[N], xr, xi);
-                    store_sum(ic<N>{}, cr, ci, xr, xi);
+                    transform_bwd<N, true>(tab, lane, R, bs[N], yr, yi);
+                    store_sum(ic<N>{}, cr, ci, yr, yi);
                 }
             }
         });
