@@ -61,7 +61,7 @@ enum {
 enum {
     SFMM_KERNELS_AUTO = 0, /* per order bucket: the autotuned family, else the default */
     SFMM_KERNELS_LANE = 1, /* one lane per translation, FP64 VALU (orders <= 12) */
-    SFMM_KERNELS_MFMA = 2, /* 16 translations per wave, swaps on FP64 MFMA (orders <= 24) */
+    SFMM_KERNELS_MFMA = 2, /* 16 translations per wave, swaps on MFMA (orders <= 24; FP32 contexts: square M2L) */
     SFMM_KERNELS_GENERIC = 3 /* one workgroup per translation, any orders <= 86 */
 };
 
@@ -98,9 +98,11 @@ size_t sfmm_max_order(void);
  * operator_handle<real>(P) (handles.hpp:40, handles.cpp:42-45): P == 0 is
  * SFMM_ERR_LOGIC, P above sfmm_max_order() (24 for the reduced modes) is
  * SFMM_ERR_OVERFLOW.  Reduced-precision contexts run square M2L of orders
- * <= 16 (FP32) or <= 10 (split) in a one-lane-per-translation kernel and
- * everything else in a one-workgroup-per-translation kernel (no MFMA
- * family, no autotuning). */
+ * <= 16 (FP32) or <= 10 (split) in a one-lane-per-translation kernel,
+ * FP32 square M2L of orders 17..24 on the matrix cores (FP32 MFMA, the
+ * kernel family SFMM_KERNELS_MFMA selects at any order up to 24), and
+ * everything else in a one-workgroup-per-translation kernel (no
+ * autotuning). */
 int sfmm_create(sfmm_context **ctx, int device, size_t P, int precision);
 void sfmm_destroy(sfmm_context *ctx);
 size_t sfmm_order(const sfmm_context *ctx);

@@ -103,9 +103,10 @@ int family_for(const sfmm_context *ctx, int op, uint32_t pb)
 
 // Kernel for one item on a context: family and whether it has one.  FP64
 // contexts: the family of its bucket max(p_in, p_out).  Reduced-precision
-// contexts: the precision lane kernel for square M2L up to kPrecLaneMax
-// (kPrecLaneMaxDW for double-word FP32) unless the generic family is
-// forced, else the precision generic kernel.
+// contexts, square M2L: the precision lane kernel up to kPrecLaneAutoMax
+// (FP32; kPrecLaneMaxDW for double-word FP32), above it the FP32 MFMA
+// kernel; everything else, and any family the context forces that has no
+// kernel for the item, the precision generic kernel.
 bool choose(const sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, int &family)
 {
     const uint32_t pb = std::max(p_in, p_out);
@@ -115,15 +116,23 @@ bool choose(const sfmm_context *ctx, int op, uint32_t p_in, uint32_t p_out, int 
         return true;
     }
     if (pb > sfmm::kPrecMaxOrder) return false;
+    const bool square_m2l = op == SFMM_M2L && p_in == p_out;
     const uint32_t lane_max = ctx->precision == SFMM_SPLIT32 ? sfmm::kPrecLaneMaxDW : sfmm::kPrecLaneMax;
-    const bool lane_ok = op == SFMM_M2L && p_in == p_out && pb <= lane_max;
+    const bool lane_ok = square_m2l && pb <= lane_max;
+    const bool mfma_ok = square_m2l && ctx->precision == SFMM_FP32;  // pb <= kPrecMaxOrder above
     switch (ctx->family) {
     case SFMM_KERNELS_LANE:
         family = sfmm::kLaneKernels;
         return lane_ok;
-    case SFMM_KERNELS_MFMA: return false;
+    case SFMM_KERNELS_MFMA:
+        family = sfmm::kMfmaKernels;
+        return mfma_ok;
     case SFMM_KERNELS_GENERIC: family = sfmm::kGenericKernel; return true;
-    default: family = lane_ok ? sfmm::kLaneKernels : sfmm::kGenericKernel; return true;
+    default:
+        if (lane_ok && (ctx->precision != SFMM_FP32 || pb <= sfmm::kPrecLaneAutoMax)) family = sfmm::kLaneKernels;
+        else if (mfma_ok) family = sfmm::kMfmaKernels;
+        else family = sfmm::kGenericKernel;
+        return true;
     }
 }
 
@@ -199,7 +208,7 @@ private:
 // are applied in list order by rounds (the k-th item of a target runs in
 // launch round k); otherwise every item writes its own scratch block and a
 // segmented reduction adds a target's blocks in list order (reduce.hip).
-// Square M2L orders 18..22 on the MFMA kernels run faster through the
+// Square FP64 M2L orders 18..22 on the MFMA kernels run faster through the
 // per-item-order variant (measured on 2e5 translations: -14 % at 18, -8 % at
 // 20, -2 % at 22; the same at 16 and 17, +13 % at 24), whose output degrees
 // ascend with fewer registers live.
@@ -326,7 +335,8 @@ int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm
         sfmm_plan::Launch &L = plan.launches.back();
         if (key.family == sfmm::kGenericKernel) {
             L.k.pk = std::max({L.k.pk, key.p_in, key.p_out});
-        } else if (key.p_in != key.pb || key.p_out != key.pb || (op == SFMM_M2L && mfma_rt_square(key))) {
+        } else if (key.p_in != key.pb || key.p_out != key.pb ||
+                   (op == SFMM_M2L && ctx->precision == SFMM_FP64 && mfma_rt_square(key))) {
             L.k.rt = true;
         }
         const sfmm_item &src_it = items[key.index];

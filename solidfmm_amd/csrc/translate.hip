@@ -2,7 +2,9 @@
 //
 // The kernels themselves live in lane_kernel.hpp (one lane per
 // translation), translate_mfma.hpp (16 translations per wave on the FP64
-// matrix cores) and generic.hip (one workgroup per translation, any order).
+// matrix cores), generic.hip (one workgroup per translation, any order) and,
+// for the reduced-precision modes, lane_prec.hpp, mfma_prec.hpp (FP32
+// matrix cores) and prec_generic.hip.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -62,10 +64,16 @@ uint32_t persistent_grid(const void *kernel, int block, size_t work)
             if (e.first.k == kernel && e.first.dev == dev) resident = e.second;
         if (!resident) {
             int per_cu = 0, cus = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+            // a failed query falls back to one workgroup per CU; its error
+            // is cleared so the launch that follows does not report it
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) {
+                (void)hipGetLastError();
                 per_cu = 1;
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+            }
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+                (void)hipGetLastError();
                 cus = 1;
+            }
             resident = uint32_t(per_cu) * uint32_t(cus);
             cache.push_back({{kernel, dev}, resident});
         }
@@ -110,6 +118,8 @@ hipError_t launch_kernel(int op, const KernelChoice &k, const sfmm_item *items, 
     if (k.precision != SFMM_FP64) {
         if (k.family == kLaneKernels && op == SFMM_M2L && !k.rt)
             return launch_prec_lane_m2l(k.precision, k.pk, items, n, src, dst, stream, acc);
+        if (k.family == kMfmaKernels && op == SFMM_M2L && !k.rt && k.precision == SFMM_FP32)
+            return launch_prec_mfma_m2l(k.pk, items, n, src, dst, stream, acc);
         if (k.family == kGenericKernel)
             return launch_prec_generic(k.precision, op, k.pk, items, n, src, dst, stream, acc);
         return hipErrorInvalidValue;

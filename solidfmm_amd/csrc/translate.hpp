@@ -63,6 +63,11 @@ hipError_t launch_prec_lane_m2l(int precision, uint32_t P, const sfmm_item *item
                                 double *dst, hipStream_t stream, int acc);
 hipError_t launch_prec_generic(int precision, int op, uint32_t pc, const sfmm_item *items, size_t n,
                                const double *src, double *dst, hipStream_t stream, int acc);
+// FP32 square M2L on the matrix cores, orders 1..kPrecMaxOrder
+// (mfma_prec.hip).  FP32 contexts use it above kPrecLaneAutoMax.
+inline constexpr uint32_t kPrecLaneAutoMax = 16;
+hipError_t launch_prec_mfma_m2l(uint32_t P, const sfmm_item *items, size_t n, const double *src, double *dst,
+                                hipStream_t stream, int acc);
 
 // One launch of a plan: family (not kAuto), the kernel's order bound pk,
 // whether items carry their own orders (rt), the arithmetic (SFMM_FP64,

@@ -88,6 +88,36 @@ def test_lane_and_generic_precision_kernels_agree(sfmm, orc, precision, P):
         assert np.percentile(errors(got, want, P), 99) <= RATIO[precision] * e64
 
 
+@pytest.mark.parametrize("P", [1, 5, 8, 13, 16, 17, 20, 24])
+def test_fp32_mfma_kernel(sfmm, orc, P):
+    """The FP32 M2L kernel on the matrix cores (orders 1..24) against the
+    FP64 oracle, at twice the ratio of unit roundoffs times the FP64
+    kernel's error on the same items, and against the one-workgroup FP32
+    kernel; a second list sends six items to every target (more than the
+    plans apply in rounds), which runs the kernel in scratch mode (scratch
+    blocks overwritten, then reduced)."""
+    n = 1000
+    items, src, dst = W.uniform_batch(W.M2L, P, P, n, 0xF32 + P)
+    blk = P * (P + 1)
+    rep = items.copy()
+    rep["dst"] = (np.arange(n, dtype=np.uint64) // 6) * np.uint64(blk)
+    for case, its in (("distinct", items), ("repeated", rep)):
+        print(f"P={P} targets={case}", flush=True)
+        got, plan = run(sfmm, W.M2L, its, src, dst, P, FP32, family=2)
+        assert all(f == 2 for f, *_ in plan.launch_info())
+        gen, _ = run(sfmm, W.M2L, its, src, dst, P, FP32, family=3)
+        want = orc.translate(W.M2L, its, src, dst.copy())
+        ref, _ = run(sfmm, W.M2L, its, src, dst, P, FP64)
+        assert np.isfinite(got).all()
+        e64, e, eg = (errors(v, want, P) for v in (ref, got, gen))
+        for q in (50, 99):
+            bound = RATIO[FP32] * max(np.percentile(e64, q), 1e-16)
+            assert np.percentile(e, q) <= bound, (case, q, np.percentile(e, q), bound)
+        # the same arithmetic as the one-workgroup kernel up to the order of
+        # the sums: the two stay within a few FP32 roundings of each other
+        assert np.percentile(e, 99) <= 4.0 * max(np.percentile(eg, 99), 2.0 ** -24)
+
+
 @pytest.mark.parametrize("precision", [FP32, SPLIT])
 def test_precision_mixed_orders(sfmm, orc, precision):
     items, src, dst = W.mixed_batch(600, 0xD00D + precision, 1, 24)
@@ -112,9 +142,15 @@ def test_precision_context_limits(sfmm):
     with pytest.raises(sfmm.OrderOverflowError):
         sfmm.Context(25, precision=FP32)
     ctx = sfmm.Context(16, precision=SPLIT)
-    ctx.set_kernels(2)  # no MFMA kernels in reduced precision
+    ctx.set_kernels(2)  # MFMA kernels: FP32 square M2L only
     items, _, _ = W.uniform_batch(W.M2L, 16, 16, 4, 1)
     with pytest.raises(sfmm.UnsupportedError):
         sfmm.Plan(ctx, W.M2L, items)
+    ctx32 = sfmm.Context(16, precision=FP32)
+    ctx32.set_kernels(2)
+    assert sfmm.Plan(ctx32, W.M2L, items).launch_info()[0][0] == 2
+    m2m, _, _ = W.uniform_batch(W.M2M, 16, 16, 4, 1, shift="octant")
+    with pytest.raises(sfmm.UnsupportedError):
+        sfmm.Plan(ctx32, W.M2M, m2m)
     with pytest.raises(sfmm.LogicError):
         sfmm.Context(4, precision=7)
