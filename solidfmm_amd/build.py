@@ -25,9 +25,9 @@ COMMON = [
 ]
 
 # Translation units; the kernel instantiations are split so they compile in
-# parallel.
-SOURCES = ["lane_prec_dw.hip", "mfma_m2l_rt.hip", "mfma_m2l.hip", "mfma_m2m_rt.hip", "mfma_l2l_rt.hip", "mfma_m2m.hip",
-           "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "mfma_prec.hip", "mfma_prec_hi.hip", "reduce.hip", "particles.hip", "translate.hip", "generic.hip", "arith_probe.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp",
+# parallel, the longest first.
+SOURCES = ["mfma_prec_hi.hip", "mfma_prec_mid.hip", "lane_prec_dw.hip", "mfma_m2l_rt.hip", "mfma_m2l.hip", "mfma_m2m_rt.hip", "mfma_l2l_rt.hip", "mfma_m2m.hip",
+           "mfma_l2l.hip", "lane_prec_f32.hip", "lane_rt.hip", "lane_fixed.hip", "prec_generic.hip", "mfma_prec.hip", "reduce.hip", "particles.hip", "translate.hip", "generic.hip", "arith_probe.hip", "swap_host.cpp", "capi.cpp", "dropin.cpp",
            "sphere.cpp", "server.cpp"]
 
 # Host-only C++ (the drop-in API over the C ABI).

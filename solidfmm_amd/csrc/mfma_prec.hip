@@ -5,18 +5,9 @@
 namespace sfmm {
 namespace mprec {
 
-template <int... Ps>
-constexpr auto make_fns(std::integer_sequence<int, Ps...>)
-{
-    struct T {
-        fn_t f[sizeof...(Ps)];
-    } t{{&k_m2l_mfma_f32<Ps + 1>...}};
-    return t;
-}
-
 fn_t kernel_lo(uint32_t P)
 {
-    static const auto fns = make_fns(std::make_integer_sequence<int, 16>{});
+    static const auto fns = make_fns<1>(std::make_integer_sequence<int, 16>{});
     return (P >= 1 && P <= 16) ? fns.f[P - 1] : nullptr;
 }
 
@@ -28,7 +19,7 @@ hipError_t launch_prec_mfma_m2l(uint32_t P, const sfmm_item *items, size_t n, co
     using namespace mprec;
     if (P < 1 || P > uint32_t(kMax) || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
-    const fn_t f = P <= 16 ? kernel_lo(P) : kernel_hi(P);
+    const fn_t f = P <= 16 ? kernel_lo(P) : P <= 20 ? kernel_mid(P) : kernel_hi(P);
     if (!f) return hipErrorInvalidValue;
     const size_t groups = (n + 15) / 16;
     const uint32_t grid =

@@ -421,10 +421,21 @@ __global__ void __launch_bounds__(64 * kWaves)
 
 using fn_t = void (*)(const double *, double *, const sfmm_item *, uint32_t, int);
 
-// Kernels of orders 1..16 (mfma_prec.hip) and 17..24 (mfma_prec_hi.hip),
-// compiled in two translation units.
+// Kernels of orders 1..16 (mfma_prec.hip), 17..20 (mfma_prec_mid.hip) and
+// 21..24 (mfma_prec_hi.hip), compiled in three translation units; each
+// returns nullptr outside its range.
 fn_t kernel_lo(uint32_t P);
+fn_t kernel_mid(uint32_t P);
 fn_t kernel_hi(uint32_t P);
+
+template <int FIRST, int... Ps>
+constexpr auto make_fns(std::integer_sequence<int, Ps...>)
+{
+    struct T {
+        fn_t f[sizeof...(Ps)];
+    } t{{&k_m2l_mfma_f32<Ps + FIRST>...}};
+    return t;
+}
 
 }  // namespace mprec
 }  // namespace sfmm
