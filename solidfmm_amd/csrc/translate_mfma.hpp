@@ -83,10 +83,25 @@ __device__ __forceinline__ dbl4 mfma(double a, double b, dbl4 c)
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// Orders whose swaps recompute their lane-dependent indices and masks at
+// every swap (OPQ).  Otherwise the compiler computes them once per kernel
+// and holds them in registers for the whole translation: about 100
+// registers at order 16.  Recomputing costs VALU instructions, so it is
+// used where the registers it frees buy something: two waves per SIMD (M2L
+// up to order 12: 8.96e8 -> 9.81e8/s at 12; M2M and L2L up to 10) or no
+// spills (from order 18: the per-item-order M2L kernels spilled 124-624
+// bytes per lane at 19-22 and spill none).  From 13 to 17 the kernels stay
+// at one wave per SIMD either way and run 2-8 % faster without it.
+template <int OP, int PI>
+constexpr bool opaque_lane()
+{
+    return PI <= (OP == SFMM_M2L ? 12 : 10) || PI >= 18;
+}
+
 // One swap step for degree N on the lane's slot values (xr, xi), with the
 // folded real and imaginary blocks (TR = false) or their transposes
 // (TR = true, via D^-1 S^T D).  tab points at the LDS copy of the table.
-template <int N, bool TR, int NSL>
+template <int N, bool TR, bool OPQ, int NSL>
 __device__ __forceinline__ void swap_mfma(const double *tab, int lane, double (&xr)[NSL], double (&xi)[NSL])
 {
     constexpr int NB = (N + 16) / 16;  // row blocks of 16 orders
@@ -94,8 +109,10 @@ __device__ __forceinline__ void swap_mfma(const double *tab, int lane, double (&
     constexpr int ST = dense_stride(N);
     const double *sr = tab + dense_base(N);
     const double *si = sr + dense_part(N);
-    const int r = lane & 15;  // operand row within the block (output order)
-    const int k = lane >> 4;  // operand column within the chunk (input order); also q
+    int ln = lane;
+    if constexpr (OPQ) asm volatile("" : "+v"(ln));  // see opaque_lane
+    const int r = ln & 15;  // operand row within the block (output order)
+    const int k = ln >> 4;  // operand column within the chunk (input order); also q
 
     if constexpr (TR) {
         // inputs of order l = 4c + q >= 1 doubled
@@ -175,23 +192,23 @@ __device__ __forceinline__ void rot(const LaneRot<NSL> &R, double scale, double 
     });
 }
 
-template <int N, bool TR, int NSL>
+template <int N, bool TR, bool OPQ, int NSL>
 __device__ __forceinline__ void transform_fwd(const double *tab, int lane, const LaneRot<NSL> &R, double scale,
                                               double (&xr)[NSL], double (&xi)[NSL])
 {
     rot<N, false, true>(R, scale, xr, xi);
-    swap_mfma<N, TR>(tab, lane, xr, xi);
+    swap_mfma<N, TR, OPQ>(tab, lane, xr, xi);
     rot<N, true, true>(R, 1.0, xr, xi);
-    swap_mfma<N, TR>(tab, lane, xr, xi);
+    swap_mfma<N, TR, OPQ>(tab, lane, xr, xi);
 }
 
-template <int N, bool TR, int NSL>
+template <int N, bool TR, bool OPQ, int NSL>
 __device__ __forceinline__ void transform_bwd(const double *tab, int lane, const LaneRot<NSL> &R, double scale,
                                               double (&xr)[NSL], double (&xi)[NSL])
 {
-    swap_mfma<N, TR>(tab, lane, xr, xi);
+    swap_mfma<N, TR, OPQ>(tab, lane, xr, xi);
     rot<N, true, false>(R, 1.0, xr, xi);
-    swap_mfma<N, TR>(tab, lane, xr, xi);
+    swap_mfma<N, TR, OPQ>(tab, lane, xr, xi);
     rot<N, false, false>(R, scale, xr, xi);
 }
 
@@ -457,7 +474,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             else fetch_old(ic<kUp ? 0 : PO - 1>{}, orr, oi);
             sched_fence();
             if (!RT || uint32_t(n) < pin_w) {
-                transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
+                transform_fwd<n, false, opaque_lane<OP, PI>()>(tab, lane, R, sc, xr, xi);
                 keep_degree(nc, xr, xi);
             } else {
                 clear_degree(nc);
@@ -493,7 +510,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                     xr[s] = ar;
                     xi[s] = ai;
                 });
-                transform_bwd<N, true>(tab, lane, R, scale, xr, xi);
+                transform_bwd<N, true, opaque_lane<OP, PI>()>(tab, lane, R, scale, xr, xi);
                 store_sum(Nc, cr, ci, xr, xi);
             }
         };
@@ -539,7 +556,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             sched_fence();
             if constexpr (n < PI) {
                 if (!RT || uint32_t(n) < pin_w) {
-                    transform_fwd<n, false>(tab, lane, R, sc, xr, xi);
+                    transform_fwd<n, false, opaque_lane<OP, PI>()>(tab, lane, R, sc, xr, xi);
                     keep_degree(ic<n>{}, xr, xi);
                 } else {
                     clear_degree(ic<n>{});
@@ -566,7 +583,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                         yr[s] = ar;
                         yi[s] = ai;
                     });
-                    transform_bwd<N, false>(tab, lane, R, sb, yr, yi);
+                    transform_bwd<N, false, opaque_lane<OP, PI>()>(tab, lane, R, sb, yr, yi);
                     store_sum(ic<N>{}, cr, ci, yr, yi);
                 }
                 sb *= r;
@@ -608,7 +625,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
             sched_fence();
             if constexpr (n >= 0) {
                 if (!RT || uint32_t(n) < pin_w) {
-                    transform_fwd<n, true>(tab, lane, R, fs[n], xr, xi);
+                    transform_fwd<n, true, opaque_lane<OP, PI>()>(tab, lane, R, fs[n], xr, xi);
                     keep_degree(ic<n>{}, xr, xi);
                 } else {
                     clear_degree(ic<n>{});
@@ -633,7 +650,7 @@ __device__ __forceinline__ void translate_group(const double *__restrict__ src_p
                         yr[s] = ar;
                         yi[s] = ai;
                     });
-                    transform_bwd<N, true>(tab, lane, R, bs[N], yr, yi);
+                    transform_bwd<N, true, opaque_lane<OP, PI>()>(tab, lane, R, bs[N], yr, yi);
                     store_sum(ic<N>{}, cr, ci, yr, yi);
                 }
             }
