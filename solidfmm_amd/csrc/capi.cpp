@@ -208,14 +208,15 @@ private:
 // are applied in list order by rounds (the k-th item of a target runs in
 // launch round k); otherwise every item writes its own scratch block and a
 // segmented reduction adds a target's blocks in list order (reduce.hip).
-// Square FP64 M2L orders 18..22 on the MFMA kernels run faster through the
-// per-item-order variant (measured on 2e5 translations: -14 % at 18, -8 % at
-// 20, -2 % at 22; the same at 16 and 17, +13 % at 24), whose output degrees
-// ascend with fewer registers live.
+// Square FP64 M2L orders 21 and 22 on the MFMA kernels run faster through
+// the per-item-order variant, whose output degrees ascend with fewer
+// registers live (2e5 translations: the fixed-order kernel is 6 % slower at
+// 22, and 1-2 % faster at 18 and 20, where this choice once saved 8-14 %
+// before the swaps recomputed their indices).
 template <class K>
 bool mfma_rt_square(const K &key)
 {
-    return key.family == sfmm::kMfmaKernels && key.pb >= 18 && key.pb <= 22;
+    return key.family == sfmm::kMfmaKernels && key.pb >= 21 && key.pb <= 22;
 }
 
 int build_plan(sfmm_context *ctx, int op, const sfmm_item *items, size_t n, sfmm_plan &plan)
